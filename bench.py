@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled 4-channel Frobenius TGV-KL primal-dual
+reconstruction on one MI355X.
+
+Workload (one "step" = one primal-dual iteration over the whole volume):
+256 x 256 x 64 slices x 4 channels (HAADF-like + 3 EDX-like) seeded
+granule phantom, 60 tilts in [-75, 75] degrees, 256 detector bins, Poisson
+data, alpha = (4, 1), mu = (0.2, 5e-4, 4e-3, 5e-4), FP32 state, inputs
+resident in HBM.  Prints one JSON line with voxel-iters/s, the roofline of
+the dominant kernel (HIP events over its launches) and a CPU baseline (the
+FP32 C/OpenMP restatement of the reference kernels on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ALPHA = (4.0, 1.0)
+MU = (0.2, 5e-4, 4e-3, 5e-4)
+KERNELS = ("dual_kernel", "radon_fwd_kernel", "radon_adj_kernel", "primal_kernel")
+
+
+def kernel_bytes(prob):
+    """Algorithmic HBM bytes per launch of each kernel: every array the kernel
+    must touch, read once and written once (neighbour reads not counted)."""
+    V = prob.voxels()                       # voxel-channels
+    B = prob.C * prob.Nz * prob.na * prob.nd  # bin-channels
+    d = 2 if prob.dim == 2 else 3
+    m = 3 if prob.dim == 2 else 6
+    tgv = prob.regularisation == "TGV"
+    # dual: read ubar, pbar, v, w; write v, w
+    dual = V * 4 * (1 + (d if tgv else 0) + 2 * d + (2 * m if tgv else 0))
+    # forward + prox: read ubar; per bin read lam, f, write lam
+    fwd = V * 4 + B * 12
+    # backprojection: read lam, write K^T lam
+    adj = B * 4 + V * 4
+    # primal: read u, ktl, v, w, p; write u, ubar, p, pbar
+    primal = V * 4 * (2 + d + 2 + ((m + 3 * d) if tgv else 0))
+    return dict(zip(KERNELS, (dual, fwd, adj, primal)))
+
+
+def cpu_baseline(f_host, angles, L, iters):
+    """Time the FP32 C/OpenMP restatement on this host (bounded sample)."""
+    from oracle import cref
+    from oracle import tgv_oracle as O
+    lib_path = None
+    try:  # native build for this host's CPU
+        tmp = tempfile.mkdtemp(prefix="cref_")
+        lib_path = os.path.join(tmp, "libcpuref_native.so")
+        subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared",
+                        "-std=c11", os.path.join(ROOT, "oracle", "cpu_ref.c"), "-o", lib_path,
+                        "-lm"], check=True, capture_output=True, timeout=120)
+        lib = cref.load(lib_path)
+        arch = "native"
+    except Exception:
+        lib = cref.load()
+        arch = "x86-64-v3"
+    C, Nz, na, nd = f_host.shape
+    ofs, _, _ = O.radon_struct((nd, nd), angles, n_detectors=nd)
+    kw = dict(alpha=ALPHA, mu=MU, L=L, coupling="frobenius", discrepancy="KL", dim=3, lib=lib)
+    t0 = time.perf_counter()
+    cref.solve(f_host, ofs, maxiter=1, **kw)
+    t1 = time.perf_counter()
+    cref.solve(f_host, ofs, maxiter=1 + iters, **kw)
+    t2 = time.perf_counter()
+    per_iter = ((t2 - t1) - (t1 - t0)) / iters
+    per_iter = per_iter if per_iter > 0 else (t2 - t1) / (1 + iters)
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    vox = C * Nz * nd * nd
+    return dict(value=vox / per_iter, unit="voxel-iters/s", cores=threads, kind="port",
+                sample=f"same 256x256x64x4 volume, {iters} iterations timed as the difference "
+                       f"of a {1 + iters}- and a 1-iteration run (allocation excluded), "
+                       f"gcc -O3 {arch} OpenMP",
+                seconds_per_iter=per_iter)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000, help="timed iterations")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-iters", type=int, default=50)
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nz", type=int, default=64)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(z-slab sharding is a design note); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from graptor_amd.phantoms import coupled_granule_problem
+    from graptor_amd.solver import TgvProblem
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    f, angles, phantom, geom = coupled_granule_problem(Nz=args.nz, device=dev)
+    prob = TgvProblem(f, geom, ALPHA, MU, dim=3, coupling="frobenius", discrepancy="KL",
+                      normest_mode="power", device=dev)
+    print(f"# problem ready: {prob.C}x{prob.Nz}x{prob.ny}x{prob.nx}, na={prob.na}, "
+          f"L={prob.k.L:.6f}", file=sys.stderr, flush=True)
+    prob.iterate(args.warmup)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    prob.iterate(args.steps)
+    end.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    gpu_ms = start.elapsed_time(end)
+    value = prob.voxels() * args.steps / wall
+    print(f"# timed {args.steps} iterations: wall {wall:.4f}s, events {gpu_ms / 1e3:.4f}s",
+          file=sys.stderr, flush=True)
+
+    # per-kernel durations: HIP events around every launch of a profiled pass
+    ms = prob.profile(args.profile_iters)
+    avg = [m / args.profile_iters for m in ms]
+    nbytes = kernel_bytes(prob)
+    kern = {k: dict(avg_ms=a, bytes=nbytes[k], gbs=nbytes[k] / (a * 1e-3) / 1e9)
+            for k, a in zip(KERNELS, avg)}
+    dom = max(kern, key=lambda k: kern[k]["avg_ms"])
+    achieved = kern[dom]["gbs"]
+    print("# kernels: " + json.dumps(kern), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        f_host = prob.f.cpu().numpy()
+        cpu = cpu_baseline(f_host, geom.angles, prob.k.L, args.cpu_iters)
+        print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
+        cpu.pop("seconds_per_iter", None)
+
+    line = {
+        "metric": "voxel-iters/s (coupled TGV-KL) + time-to-1e-3 rel err: warm-start vs plain PD",
+        "value": value,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded granule phantom, Poisson counts)",
+        "config": {"workload": "3D Frobenius-coupled 4-channel TGV-KL, 256x256x%d slices, "
+                               "60 tilts in [-75,75] deg, 256 bins" % args.nz,
+                   "volume": [4, args.nz, 256, 256], "angles": 60, "bins": 256,
+                   "alpha": list(ALPHA), "mu": list(MU), "iterations_timed": args.steps},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None,
+                     "bytes_per_launch": kern[dom]["bytes"],
+                     "avg_launch_ms": kern[dom]["avg_ms"]},
+        "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,642 @@
+// TGV stencil kernels for gfx950.
+//
+// dual_kernel   : one thread per voxel, all channels: v += sigma (grad ubar - pbar),
+//                 w += sigma E(pbar), then the coupled projections of v and w
+//                 (update_v/update_w/update_NormV_*/update_NormW_* of
+//                 radon_tgv_primal_dual_2d.py 2D:98-304 and _3d.py 3D:290-660
+//                 fused into one pass).
+// primal_kernel : one thread per voxel-channel: u+ = max(0, u + tau (div v -
+//                 K^T lam / L)), p+ = p + tau (v + div2 w) and the
+//                 extrapolations ubar = 2u+ - u, pbar = 2p+ - p
+//                 (update_u/update_p/update_extra 2D:340-376, 2D:637-638).
+// energy kernels: FP64 block partial sums of the primal energy.
+#include "common.h"
+
+#include <cmath>
+#include <vector>
+
+namespace gr {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+constexpr float W_OFF = 1.4142f;  // the reference's float literal for sqrt(2)
+
+// ---- closed-form projection helpers -------------------------------------
+
+// 2x2 Gram route of update_NormV_Nucl (2D:124-204): M = sum_k S_k e_k e_k^T
+// with S_k = min(1, 1/(alphainv sigma_k)); G treated as diagonal when its
+// off-diagonal is negligible against the smaller diagonal entry.
+__device__ __forceinline__ void nucl2d(float a, float b, float c, float ainv, float& m00,
+                                       float& m01, float& m11)
+{
+    const float eps = 1e-5f;
+    const float o = fminf(a, c);
+    if ((b * b) / (o * o) > eps && o * o > 0.f) {
+        const float d = a + c, e = a - c;
+        float f = 4.f * b * b + e * e;
+        f = f < 0.f ? 0.f : sqrtf(f);
+        const float s2 = d - f > 0.f ? sqrtf((d - f) * 0.5f) : 0.f;
+        const float s1 = d + f > 0.f ? sqrtf((d + f) * 0.5f) : 0.f;
+        const float g = (e + f) * 0.5f, hh = (e - f) * 0.5f;
+        float v0 = g * a + b * b, v1 = g * b + c * b;
+        float v2 = hh * a + b * b, v3 = hh * b + b * c;
+        const float n1 = sqrtf(v0 * v0 + v1 * v1), n2 = sqrtf(v2 * v2 + v3 * v3);
+        if (n1 > 0.f) { v0 /= n1; v1 /= n1; }
+        if (n2 > 0.f) { v2 /= n2; v3 /= n2; }
+        const float S1 = s1 * ainv > 1.f ? 1.f / (s1 * ainv) : 1.f;
+        const float S2 = s2 * ainv > 1.f ? 1.f / (s2 * ainv) : 1.f;
+        m00 = v0 * v0 * S1 + v2 * v2 * S2;
+        m11 = v1 * v1 * S1 + v3 * v3 * S2;
+        m01 = S1 * v0 * v1 + S2 * v2 * v3;
+    } else {
+        const float s1 = sqrtf(a), s2 = sqrtf(c);
+        m00 = ainv * s1 > 1.f ? 1.f / (ainv * s1) : 1.f;
+        m11 = ainv * s2 > 1.f ? 1.f / (ainv * s2) : 1.f;
+        m01 = 0.f;
+    }
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix
+// A = (a00, a11, a22, a01, a02, a12): ev[k], eigenvector k in column k of V.
+template <typename T>
+__device__ __forceinline__ void jacobi3(const T (&A)[6], T (&ev)[3], T (&V)[3][3])
+{
+    T a[3][3] = {{A[0], A[3], A[4]}, {A[3], A[1], A[5]}, {A[4], A[5], A[2]}};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) V[r][k] = (r == k) ? T(1) : T(0);
+#pragma unroll
+    for (int sweep = 0; sweep < 6; ++sweep) {
+#pragma unroll
+        for (int pq = 0; pq < 3; ++pq) {
+            const int P = (pq == 2) ? 1 : 0, Q = (pq == 0) ? 1 : 2;
+            const T apq = a[P][Q];
+            if (apq == T(0)) continue;
+            const T theta = (a[Q][Q] - a[P][P]) / (T(2) * apq);
+            const T t = copysign(T(1), theta) / (fabs(theta) + sqrt(theta * theta + T(1)));
+            const T cs = T(1) / sqrt(t * t + T(1)), sn = t * cs;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const T akp = a[k][P], akq = a[k][Q];
+                a[k][P] = cs * akp - sn * akq;
+                a[k][Q] = sn * akp + cs * akq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const T apk = a[P][k], aqk = a[Q][k];
+                a[P][k] = cs * apk - sn * aqk;
+                a[Q][k] = sn * apk + cs * aqk;
+            }
+            a[P][Q] = a[Q][P] = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const T vkp = V[k][P], vkq = V[k][Q];
+                V[k][P] = cs * vkp - sn * vkq;
+                V[k][Q] = sn * vkp + cs * vkq;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ev[k] = a[k][k];
+}
+
+// ---- dual step ------------------------------------------------------------
+
+template <int DIM, int CPL, bool TGV, int MAXC>
+__global__ __launch_bounds__(BX* BY) void dual_kernel(
+    float* __restrict__ v, float* __restrict__ w, const float* __restrict__ ubar,
+    const float* __restrict__ pbar, int nx, int ny, int nz, int nch, float hinv,
+    float sigma, float ainv0, float ainv1)
+{
+    constexpr int D = DIM;
+    constexpr int M = DIM == 2 ? 3 : 6;
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    const size_t plane = (size_t)nx * ny;
+    const size_t chs = plane * nz;        // channel stride
+    const size_t cs = chs * nch;          // component stride
+    const size_t i = (size_t)z * plane + (size_t)y * nx + x;
+
+    float vv[MAXC][D];
+    float ww[MAXC][TGV ? M : 1];
+    float nv2 = 0.f, nw2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+        if (c >= nch) break;
+        const size_t ic = i + c * chs;
+        const float u0 = ubar[ic];
+        float g[3];
+        g[0] = (x < nx - 1) ? ubar[ic + 1] - u0 : 0.f;
+        g[1] = (y < ny - 1) ? ubar[ic + nx] - u0 : 0.f;
+        if constexpr (DIM == 3) g[2] = (z < nz - 1) ? (ubar[ic + plane] - u0) * hinv : 0.f;
+        float p[3] = {0.f, 0.f, 0.f};
+        if constexpr (TGV) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) p[k] = pbar[ic + k * cs];
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            vv[c][k] = v[ic + k * cs] + sigma * (g[k] - p[k]);
+            nv2 += vv[c][k] * vv[c][k];
+        }
+        if constexpr (TGV) {
+            float ax[3], ay[3], az[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                ax[k] = (x > 0) ? p[k] - pbar[ic - 1 + k * cs] : 0.f;
+                ay[k] = (y > 0) ? p[k] - pbar[ic - nx + k * cs] : 0.f;
+                if constexpr (DIM == 3)
+                    az[k] = (z > 0) ? (p[k] - pbar[ic - plane + k * cs]) * hinv : 0.f;
+            }
+            float e[6];
+            e[0] = ax[0];
+            e[1] = ay[1];
+            if constexpr (DIM == 2) {
+                e[2] = 0.5f * (ax[1] + ay[0]);
+            } else {
+                e[2] = az[2];
+                e[3] = 0.5f * (ax[1] + ay[0]);
+                e[4] = 0.5f * (ax[2] + az[0]);
+                e[5] = 0.5f * (ay[2] + az[1]);
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) ww[c][m] = w[ic + m * cs] + sigma * e[m];
+            // weighted squared norm of this channel's tensor (2D:232, 3D:582)
+            float n2 = 0.f;
+            if constexpr (DIM == 2) {
+                const float od = CPL == GR_COUPLING_UNCORRELATED ? W_OFF * ww[c][2]
+                                                                 : 2.f * ww[c][2];
+                n2 = ww[c][0] * ww[c][0] + ww[c][1] * ww[c][1] + od * od;
+            } else {
+                const float o3 = W_OFF * ww[c][3], o4 = W_OFF * ww[c][4];
+                const float o5 = CPL == GR_COUPLING_UNCORRELATED ? ww[c][5] : W_OFF * ww[c][5];
+                n2 = ww[c][0] * ww[c][0] + ww[c][1] * ww[c][1] + ww[c][2] * ww[c][2] +
+                     o3 * o3 + o4 * o4 + o5 * o5;
+            }
+            if constexpr (CPL == GR_COUPLING_UNCORRELATED) {
+                const float n = sqrtf(n2) * ainv0;
+                const float sc = n > 1.f ? 1.f / n : 1.f;
+#pragma unroll
+                for (int m = 0; m < M; ++m) ww[c][m] *= sc;
+            } else {
+                nw2 += n2;
+            }
+        }
+        if constexpr (CPL == GR_COUPLING_UNCORRELATED) {
+            float n2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) n2 += vv[c][k] * vv[c][k];
+            const float n = sqrtf(n2) * ainv1;
+            const float sc = n > 1.f ? 1.f / n : 1.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) vv[c][k] *= sc;
+        }
+    }
+
+    // coupled projections
+    if constexpr (CPL == GR_COUPLING_FROBENIUS) {
+        const float n = sqrtf(nv2) * ainv1;
+        const float sc = n > 1.f ? 1.f / n : 1.f;
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+#pragma unroll
+            for (int k = 0; k < D; ++k) vv[c][k] *= sc;
+    } else if constexpr (CPL == GR_COUPLING_NUCLEAR) {
+        if constexpr (DIM == 2) {
+            float a = 0.f, b = 0.f, c2 = 0.f;
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+                if (c >= nch) break;
+                a += vv[c][0] * vv[c][0];
+                b += vv[c][0] * vv[c][1];
+                c2 += vv[c][1] * vv[c][1];
+            }
+            float m00, m01, m11;
+            nucl2d(a, b, c2, ainv1, m00, m01, m11);
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+                const float x0 = vv[c][0], x1 = vv[c][1];
+                vv[c][0] = x0 * m00 + x1 * m01;
+                vv[c][1] = x0 * m01 + x1 * m11;
+            }
+        } else {
+            float G[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+                if (c >= nch) break;
+                G[0] += vv[c][0] * vv[c][0];
+                G[1] += vv[c][1] * vv[c][1];
+                G[2] += vv[c][2] * vv[c][2];
+                G[3] += vv[c][0] * vv[c][1];
+                G[4] += vv[c][0] * vv[c][2];
+                G[5] += vv[c][1] * vv[c][2];
+            }
+            float ev[3], V[3][3], S[3];
+            jacobi3<float>(G, ev, V);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float sg = ev[k] > 0.f ? sqrtf(ev[k]) : 0.f;
+                S[k] = sg * ainv1 > 1.f ? 1.f / (ainv1 * sg) : 1.f;
+            }
+            float Mx[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    Mx[r][q] = S[0] * V[r][0] * V[q][0] + S[1] * V[r][1] * V[q][1] +
+                               S[2] * V[r][2] * V[q][2];
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+                const float a0 = vv[c][0], a1 = vv[c][1], a2 = vv[c][2];
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    vv[c][q] = a0 * Mx[0][q] + a1 * Mx[1][q] + a2 * Mx[2][q];
+            }
+        }
+    }
+    float wsc = 1.f;
+    if constexpr (TGV && CPL != GR_COUPLING_UNCORRELATED) {
+        const float n = sqrtf(nw2) * ainv0;
+        wsc = n > 1.f ? 1.f / n : 1.f;
+    }
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+        if (c >= nch) break;
+        const size_t ic = i + c * chs;
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[ic + k * cs] = vv[c][k];
+        if constexpr (TGV) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) w[ic + m * cs] = ww[c][m] * wsc;
+        }
+    }
+}
+
+// ---- primal step + extrapolation ----------------------------------------
+
+template <int DIM, bool TGV>
+__global__ __launch_bounds__(BX* BY) void primal_kernel(
+    float* __restrict__ u, float* __restrict__ ubar, float* __restrict__ p,
+    float* __restrict__ pbar, const float* __restrict__ v, const float* __restrict__ w,
+    const float* __restrict__ ktl, int nx, int ny, int nz, int nch, float hinv, float tau,
+    float norming)
+{
+    const int x = blockIdx.x * BX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int zc = blockIdx.z;  // z + nz * channel
+    if (x >= nx || y >= ny) return;
+    const int z = zc % nz;
+    const size_t plane = (size_t)nx * ny;
+    const size_t cs = plane * nz * nch;
+    const size_t i = (size_t)zc * plane + (size_t)y * nx + x;
+
+    float vc[3];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
+    float d0 = (x == nx - 1) ? 0.f : vc[0];
+    if (x > 0) d0 -= v[i - 1];
+    float d1 = (y == ny - 1) ? 0.f : vc[1];
+    if (y > 0) d1 -= v[i + cs - nx];
+    float dv = d0 + d1;
+    if constexpr (DIM == 3) {
+        float d2 = (z == nz - 1) ? 0.f : vc[2];
+        if (z > 0) d2 -= v[i + 2 * cs - plane];
+        dv += d2 * hinv;
+    }
+    const float uo = u[i];
+    float un = uo + tau * (dv - norming * ktl[i]);
+    un = un < 0.f ? 0.f : un;
+    u[i] = un;
+    ubar[i] = 2.f * un - uo;
+
+    if constexpr (TGV) {
+        constexpr int M = DIM == 2 ? 3 : 6;
+        float wc[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) wc[m] = w[i + m * cs];
+        float q[3];
+        if constexpr (DIM == 2) {
+            // x-part uses (w_xx, w_xy), y-part (w_xy, w_yy)
+            float x0 = (x == 0) ? 0.f : -wc[0], x1 = (x == 0) ? 0.f : -wc[2];
+            if (x < nx - 1) { x0 += w[i + 1]; x1 += w[i + 2 * cs + 1]; }
+            float y0 = (y == 0) ? 0.f : -wc[2], y1 = (y == 0) ? 0.f : -wc[1];
+            if (y < ny - 1) { y0 += w[i + 2 * cs + nx]; y1 += w[i + cs + nx]; }
+            q[0] = x0 + y0;
+            q[1] = x1 + y1;
+        } else {
+            // tensor order (xx, yy, zz, xy, xz, yz): x-part (0,3,4), y-part (3,1,5), z-part (4,5,2)
+            const int xi[3] = {0, 3, 4}, yi[3] = {3, 1, 5}, zi[3] = {4, 5, 2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float xs = (x == 0) ? 0.f : -wc[xi[k]];
+                if (x < nx - 1) xs += w[i + xi[k] * cs + 1];
+                float ys = (y == 0) ? 0.f : -wc[yi[k]];
+                if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
+                float zs = (z == 0) ? 0.f : -wc[zi[k]];
+                if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
+                q[k] = xs + ys + zs * hinv;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            const float po = p[i + k * cs];
+            const float pn = po + tau * (vc[k] + q[k]);
+            p[i + k * cs] = pn;
+            pbar[i + k * cs] = 2.f * pn - po;
+        }
+    }
+}
+
+// ---- FP64 energy ----------------------------------------------------------
+
+constexpr int EN_THREADS = 256;
+
+__device__ double block_sum(double v, double* red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = EN_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// Regulariser a1 |grad u - p|_V* + a0 |E p|_W*, pointwise dual norms of the
+// projection balls; one block partial per block.
+template <int DIM, int CPL, bool TGV>
+__global__ __launch_bounds__(EN_THREADS) void reg_energy_kernel(
+    const float* __restrict__ u, const float* __restrict__ p, int nx, int ny, int nz,
+    int nch, double hinv, double a0, double a1, double* __restrict__ partial)
+{
+    __shared__ double red[EN_THREADS];
+    constexpr int D = DIM;
+    constexpr int M = DIM == 2 ? 3 : 6;
+    const size_t plane = (size_t)nx * ny;
+    const size_t nvox = plane * nz;
+    const size_t chs = nvox, cs = nvox * nch;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * EN_THREADS + threadIdx.x; i < nvox;
+         i += (size_t)gridDim.x * EN_THREADS) {
+        const int x = (int)(i % nx);
+        const int y = (int)((i / nx) % ny);
+        const int z = (int)(i / plane);
+        double fro_v = 0.0, fro_w = 0.0, sum_v = 0.0, sum_w = 0.0;
+        double G[6] = {0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < nch; ++c) {
+            const size_t ic = i + c * chs;
+            const double u0 = u[ic];
+            double q[3];
+            q[0] = (x < nx - 1) ? (double)u[ic + 1] - u0 : 0.0;
+            q[1] = (y < ny - 1) ? (double)u[ic + nx] - u0 : 0.0;
+            q[2] = (DIM == 3 && z < nz - 1) ? ((double)u[ic + plane] - u0) * hinv : 0.0;
+            double pc[3] = {0, 0, 0};
+            if (TGV)
+                for (int k = 0; k < D; ++k) pc[k] = p[ic + k * cs];
+            for (int k = 0; k < D; ++k) q[k] -= pc[k];
+            double n2 = 0.0;
+            for (int k = 0; k < D; ++k) n2 += q[k] * q[k];
+            fro_v += n2;
+            sum_v += sqrt(n2);
+            G[0] += q[0] * q[0];
+            G[1] += q[1] * q[1];
+            G[2] += q[2] * q[2];
+            G[3] += q[0] * q[1];
+            G[4] += q[0] * q[2];
+            G[5] += q[1] * q[2];
+            if (TGV) {
+                double ax[3] = {0, 0, 0}, ay[3] = {0, 0, 0}, az[3] = {0, 0, 0};
+                for (int k = 0; k < D; ++k) {
+                    ax[k] = x > 0 ? pc[k] - (double)p[ic - 1 + k * cs] : 0.0;
+                    ay[k] = y > 0 ? pc[k] - (double)p[ic - nx + k * cs] : 0.0;
+                    if (DIM == 3)
+                        az[k] = z > 0 ? (pc[k] - (double)p[ic - plane + k * cs]) * hinv : 0.0;
+                }
+                double e[6];
+                e[0] = ax[0];
+                e[1] = ay[1];
+                double cw[6];  // b_k^2 / a_k of the dual norm
+                if (DIM == 2) {
+                    e[2] = 0.5 * (ax[1] + ay[0]);
+                    const double od = CPL == GR_COUPLING_UNCORRELATED
+                                          ? (double)W_OFF * W_OFF : 4.0;
+                    cw[0] = 1; cw[1] = 1; cw[2] = 4.0 / od;
+                } else {
+                    e[2] = az[2];
+                    e[3] = 0.5 * (ax[1] + ay[0]);
+                    e[4] = 0.5 * (ax[2] + az[0]);
+                    e[5] = 0.5 * (ay[2] + az[1]);
+                    const double r2 = (double)W_OFF * W_OFF;
+                    cw[0] = 1; cw[1] = 1; cw[2] = 1;
+                    cw[3] = 4.0 / r2; cw[4] = 4.0 / r2;
+                    cw[5] = CPL == GR_COUPLING_UNCORRELATED ? 4.0 : 4.0 / r2;
+                }
+                double m2 = 0.0;
+                for (int m = 0; m < M; ++m) m2 += cw[m] * e[m] * e[m];
+                fro_w += m2;
+                sum_w += sqrt(m2);
+            }
+        }
+        double rv;
+        if (CPL == GR_COUPLING_UNCORRELATED) {
+            rv = sum_v;
+        } else if (CPL == GR_COUPLING_FROBENIUS) {
+            rv = sqrt(fro_v);
+        } else {
+            double ev[3], V[3][3];
+            jacobi3<double>(G, ev, V);
+            rv = 0.0;
+            for (int k = 0; k < (DIM == 2 ? 3 : 3); ++k) rv += ev[k] > 0 ? sqrt(ev[k]) : 0.0;
+        }
+        double rw = 0.0;
+        if (TGV) rw = (CPL == GR_COUPLING_UNCORRELATED) ? sum_w : sqrt(fro_w);
+        acc += a1 * rv + a0 * rw;
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Data term per channel: KL(y, f) = y - f + f log(f / y) (f > 0; +inf when
+// y <= 0 < f), y when f == 0; L2: (y - f)^2 / 2.  partial[c * nb + b].
+__global__ __launch_bounds__(EN_THREADS) void data_energy_kernel(
+    const float* __restrict__ ku, const float* __restrict__ f, size_t per_ch, int disc,
+    double* __restrict__ partial)
+{
+    __shared__ double red[EN_THREADS];
+    const int c = blockIdx.y;
+    const float* kc = ku + (size_t)c * per_ch;
+    const float* fc = f + (size_t)c * per_ch;
+    double acc = 0.0;
+    for (size_t i = (size_t)blockIdx.x * EN_THREADS + threadIdx.x; i < per_ch;
+         i += (size_t)gridDim.x * EN_THREADS) {
+        const double yv = kc[i], fv = fc[i];
+        if (disc == GR_DISC_KL) {
+            if (fv > 0.0)
+                acc += yv > 0.0 ? yv - fv + fv * log(fv / yv) : INFINITY;
+            else
+                acc += yv;
+        } else {
+            acc += 0.5 * (yv - fv) * (yv - fv);
+        }
+    }
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = s;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+
+template <int DIM, int CPL, bool TGV>
+static int dual_dispatch_c(const TgvDims& d, float* v, float* w, const float* ubar,
+                           const float* pbar, float sigma, float ainv0, float ainv1,
+                           hipStream_t st)
+{
+    dim3 blk(BX, BY), grd((d.nx + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz);
+    const float hinv = 1.f / d.h;
+#define GR_DUAL(MC)                                                                        \
+    hipLaunchKernelGGL((dual_kernel<DIM, CPL, TGV, MC>), grd, blk, 0, st, v, w, ubar, pbar, \
+                       d.nx, d.ny, d.nz, d.nch, hinv, sigma, ainv0, ainv1)
+    if (d.nch <= 1) GR_DUAL(1);
+    else if (d.nch <= 2) GR_DUAL(2);
+    else if (d.nch <= 4) GR_DUAL(4);
+    else GR_DUAL(8);
+#undef GR_DUAL
+    GR_LAUNCH_CHECK("dual_kernel");
+    return GR_OK;
+}
+
+template <int DIM, int CPL>
+static int dual_dispatch_t(const TgvDims& d, float* v, float* w, const float* ubar,
+                           const float* pbar, float sigma, float ainv0, float ainv1,
+                           hipStream_t st)
+{
+    if (d.tgv) return dual_dispatch_c<DIM, CPL, true>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    return dual_dispatch_c<DIM, CPL, false>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+}
+
+template <int DIM>
+static int dual_dispatch_d(const TgvDims& d, float* v, float* w, const float* ubar,
+                           const float* pbar, float sigma, float ainv0, float ainv1,
+                           hipStream_t st)
+{
+    switch (d.coupling) {
+    case GR_COUPLING_UNCORRELATED:
+        return dual_dispatch_t<DIM, GR_COUPLING_UNCORRELATED>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    case GR_COUPLING_FROBENIUS:
+        return dual_dispatch_t<DIM, GR_COUPLING_FROBENIUS>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    case GR_COUPLING_NUCLEAR:
+        return dual_dispatch_t<DIM, GR_COUPLING_NUCLEAR>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    }
+    set_error("unknown coupling %d", d.coupling);
+    return GR_ERR_INVALID;
+}
+
+int launch_dual(const TgvDims& d, float* v, float* w, const float* ubar, const float* pbar,
+                float sigma, float ainv0, float ainv1, hipStream_t st)
+{
+    if (d.nch > 8) {
+        set_error("dual step: %d channels exceed the 8 this build holds per voxel", d.nch);
+        return GR_ERR_UNSUPPORTED;
+    }
+    if (d.dim == 2) return dual_dispatch_d<2>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    return dual_dispatch_d<3>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+}
+
+int launch_primal(const TgvDims& d, float* u, float* ubar, float* p, float* pbar,
+                  const float* v, const float* w, const float* ktl, float tau, float norming,
+                  hipStream_t st)
+{
+    dim3 blk(BX, BY), grd((d.nx + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz * d.nch);
+    const float hinv = 1.f / d.h;
+#define GR_PRIMAL(DM, TG)                                                                    \
+    hipLaunchKernelGGL((primal_kernel<DM, TG>), grd, blk, 0, st, u, ubar, p, pbar, v, w, ktl, \
+                       d.nx, d.ny, d.nz, d.nch, hinv, tau, norming)
+    if (d.dim == 2) {
+        if (d.tgv) GR_PRIMAL(2, true); else GR_PRIMAL(2, false);
+    } else {
+        if (d.tgv) GR_PRIMAL(3, true); else GR_PRIMAL(3, false);
+    }
+#undef GR_PRIMAL
+    GR_LAUNCH_CHECK("primal_kernel");
+    return GR_OK;
+}
+
+template <int DIM, int CPL, bool TGV>
+static void reg_launch(const TgvDims& d, const float* u, const float* p, double a0, double a1,
+                       double* partial, int nb, hipStream_t st)
+{
+    hipLaunchKernelGGL((reg_energy_kernel<DIM, CPL, TGV>), dim3(nb), dim3(EN_THREADS), 0, st, u,
+                       p, d.nx, d.ny, d.nz, d.nch, 1.0 / (double)d.h, a0, a1, partial);
+}
+
+int launch_energy(const TgvDims& d, const float* u, const float* p, double a0, double a1,
+                  double* out_reg, hipStream_t st)
+{
+    const int nb = 1024;
+    double* partial = nullptr;
+    if (hipMalloc(&partial, sizeof(double) * nb) != hipSuccess) {
+        set_error("energy: device allocation failed");
+        return GR_ERR_OOM;
+    }
+#define GR_REG(DM, CP)                                                             \
+    do {                                                                           \
+        if (d.tgv) reg_launch<DM, CP, true>(d, u, p, a0, a1, partial, nb, st);     \
+        else reg_launch<DM, CP, false>(d, u, p, a0, a1, partial, nb, st);          \
+    } while (0)
+    if (d.dim == 2) {
+        if (d.coupling == GR_COUPLING_UNCORRELATED) GR_REG(2, GR_COUPLING_UNCORRELATED);
+        else if (d.coupling == GR_COUPLING_FROBENIUS) GR_REG(2, GR_COUPLING_FROBENIUS);
+        else GR_REG(2, GR_COUPLING_NUCLEAR);
+    } else {
+        if (d.coupling == GR_COUPLING_UNCORRELATED) GR_REG(3, GR_COUPLING_UNCORRELATED);
+        else if (d.coupling == GR_COUPLING_FROBENIUS) GR_REG(3, GR_COUPLING_FROBENIUS);
+        else GR_REG(3, GR_COUPLING_NUCLEAR);
+    }
+#undef GR_REG
+    std::vector<double> h(nb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(partial);
+    if (e != hipSuccess) return hip_status(e, "energy");
+    double s = 0.0;
+    for (double x : h) s += x;
+    *out_reg = s;
+    return GR_OK;
+}
+
+int launch_data_energy(const float* ku, const float* f, const double* mu, int nch,
+                       size_t per_ch, int disc, double* out, hipStream_t st)
+{
+    const int nb = 256;
+    double* partial = nullptr;
+    if (hipMalloc(&partial, sizeof(double) * nb * nch) != hipSuccess) {
+        set_error("energy: device allocation failed");
+        return GR_ERR_OOM;
+    }
+    hipLaunchKernelGGL(data_energy_kernel, dim3(nb, nch), dim3(EN_THREADS), 0, st, ku, f, per_ch,
+                       disc, partial);
+    std::vector<double> h((size_t)nb * nch);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h.data(), partial, sizeof(double) * nb * nch, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(partial);
+    if (e != hipSuccess) return hip_status(e, "data energy");
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        double sc = 0.0;
+        for (int b = 0; b < nb; ++b) sc += h[(size_t)c * nb + b];
+        s += mu[c] * sc;
+    }
+    *out = s;
+    return GR_OK;
+}
+
+}  // namespace gr

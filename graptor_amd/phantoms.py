@@ -1,0 +1,119 @@
+"""Seeded synthetic phantoms and data for the benchmark configurations.
+
+All generators use ``numpy.random.default_rng(seed)``; projections of the
+phantoms are computed on the GPU with this package's own projector.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .projector import RadonGeometry, radon
+
+
+def tilt_angles(lo_deg, hi_deg, n, endpoint=True):
+    """Tilt series as solver angles: theta + pi/2, as the reference's caller
+    passes them (Reconstruction_coupled.py:233, 448)."""
+    return np.deg2rad(np.linspace(lo_deg, hi_deg, n, endpoint=endpoint)) + np.pi / 2
+
+
+def ellipse_phantom(rng, n_support, n_grid, n_min=8, n_max=15):
+    """2D phantom: n_min..n_max random ellipses with intensities U[0.1, 1],
+    additive, clipped >= 0, on an n_support square centred in n_grid."""
+    img = np.zeros((n_grid, n_grid))
+    off = (n_grid - n_support) // 2
+    yy, xx = np.mgrid[0:n_support, 0:n_support] - (n_support - 1) / 2.0
+    sub = np.zeros((n_support, n_support))
+    for _ in range(rng.integers(n_min, n_max + 1)):
+        cx, cy = rng.uniform(-0.3, 0.3, 2) * n_support
+        a, b = rng.uniform(0.05, 0.3, 2) * n_support
+        th = rng.uniform(0, np.pi)
+        xr = (xx - cx) * np.cos(th) + (yy - cy) * np.sin(th)
+        yr = -(xx - cx) * np.sin(th) + (yy - cy) * np.cos(th)
+        sub += rng.uniform(0.1, 1.0) * ((xr / a) ** 2 + (yr / b) ** 2 < 1.0)
+    img[off:off + n_support, off:off + n_support] = np.maximum(sub, 0.0)
+    return img
+
+
+def granule_volume(rng, N, Nz, n_maps=3, n_min=20, n_max=60):
+    """3D granule phantom: a cylindrical matrix holding ``n_maps`` element
+    maps of n_min..n_max ellipsoidal granules each.  Returns (matrix, maps)
+    with maps shaped (n_maps, Nz, N, N), piecewise constant."""
+    z, y, x = np.ogrid[0:Nz, 0:N, 0:N]
+    cy = cx = (N - 1) / 2.0
+    rad = 0.42 * N
+    matrix = (((x - cx) ** 2 + (y - cy) ** 2) < rad ** 2) * np.ones((Nz, 1, 1))
+    maps = np.zeros((n_maps, Nz, N, N))
+    for k in range(n_maps):
+        for _ in range(rng.integers(n_min, n_max + 1)):
+            r = rad * np.sqrt(rng.uniform(0, 0.8))
+            phi = rng.uniform(0, 2 * np.pi)
+            gx, gy = cx + r * np.cos(phi), cy + r * np.sin(phi)
+            gz = rng.uniform(0, Nz - 1)
+            ax = rng.uniform(0.02, 0.08, 3) * np.array([N, N, max(Nz, 8)])
+            lo = [max(0, int(gz - ax[2]) - 1), max(0, int(gy - ax[1]) - 1),
+                  max(0, int(gx - ax[0]) - 1)]
+            hi = [min(Nz, int(gz + ax[2]) + 2), min(N, int(gy + ax[1]) + 2),
+                  min(N, int(gx + ax[0]) + 2)]
+            zz = np.arange(lo[0], hi[0])[:, None, None]
+            yy = np.arange(lo[1], hi[1])[None, :, None]
+            xx = np.arange(lo[2], hi[2])[None, None, :]
+            inside = (((xx - gx) / ax[0]) ** 2 + ((yy - gy) / ax[1]) ** 2
+                      + ((zz - gz) / ax[2]) ** 2) < 1.0
+            val = rng.uniform(0.5, 1.0)
+            blk = maps[k, lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+            np.maximum(blk, val * inside, out=blk)
+    return matrix, maps
+
+
+def coupled_granule_problem(N=256, Nz=64, na=60, tilt=(-75.0, 75.0), seed_phantom=2001,
+                            seed_noise=2002, haadf_counts=1e4, edx_counts=5.0,
+                            device="cuda"):
+    """4-channel HAADF + 3 x EDX problem of granules in a matrix.
+
+    Channel 0 = weighted sum of matrix and element maps scaled to a mean of
+    ``haadf_counts`` per illuminated bin, channels 1..3 = the element maps
+    scaled to ``edx_counts``; Poisson noise.  Returns (f (4, Nz, na, N)
+    float32 CUDA tensor, angles, phantom (4, Nz, N, N) float32 numpy).
+    """
+    rng = np.random.default_rng(seed_phantom)
+    matrix, maps = granule_volume(rng, N, Nz)
+    haadf = 0.25 * matrix + 1.0 * maps[0] + 0.6 * maps[1] + 0.8 * maps[2]
+    phantom = np.stack([haadf, maps[0], maps[1], maps[2]]).astype(np.float32)
+    angles = tilt_angles(tilt[0], tilt[1], na)
+    geom = RadonGeometry((N, N), angles, n_detectors=N)
+    img = torch.as_tensor(phantom, device=device)
+    sino = torch.zeros((4, Nz, na, N), device=device)
+    radon(sino.view(4 * Nz, na, N), img.view(4 * Nz, N, N), geom)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed_noise)
+    f = torch.empty_like(sino)
+    targets = [haadf_counts, edx_counts, edx_counts, edx_counts]
+    scales = []
+    for c in range(4):
+        s = sino[c]
+        mean = s[s > 0].mean()
+        scale = float(targets[c] / mean) if mean > 0 else 1.0
+        scales.append(scale)
+        f[c] = torch.poisson(torch.clamp(s * scale, min=0), generator=gen)
+    phantom_scaled = phantom * np.array(scales, np.float32)[:, None, None, None]
+    return f, angles, phantom_scaled, geom
+
+
+def ellipse_problem_2d(N_support=128, N_grid=183, na=90, lo=-70.0, hi=70.0, seed_phantom=1001,
+                       seed_noise=1002, noise=0.01, device="cuda"):
+    """Single-channel 2D problem: ellipse phantom, Gaussian noise of
+    ``noise`` x max.  Returns (f (1, 1, na, N_grid) CUDA tensor, angles,
+    phantom, geometry)."""
+    rng = np.random.default_rng(seed_phantom)
+    ph = ellipse_phantom(rng, N_support, N_grid).astype(np.float32)
+    angles = tilt_angles(lo, hi, na, endpoint=False)
+    geom = RadonGeometry((N_grid, N_grid), angles, n_detectors=N_grid)
+    img = torch.as_tensor(ph, device=device).view(1, N_grid, N_grid)
+    sino = torch.zeros((1, na, N_grid), device=device)
+    radon(sino, img, geom)
+    nrng = np.random.default_rng(seed_noise)
+    s = sino.cpu().numpy()
+    s = s + nrng.normal(0.0, noise * s.max(), s.shape)
+    return (torch.as_tensor(s.astype(np.float32), device=device).view(1, 1, na, N_grid), angles,
+            ph, geom)

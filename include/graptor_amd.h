@@ -1,0 +1,139 @@
+/*
+ * graptor_amd: C ABI of the MI355X-native primal-dual TGV reconstruction path.
+ *
+ * Plain C: caller-owned device pointers (e.g. torch.Tensor.data_ptr()),
+ * sizes and a hipStream_t passed as void*.  Every function returns an int
+ * status (GR_OK == 0) and never throws; gr_last_error() describes the last
+ * failure of the calling thread.  Device arrays use the reference's
+ * pyopencl order='F' layouts (first index fastest):
+ *   image stack  u[c][z][y][x]            (2D: nz == 1)
+ *   vector field v[k][c][z][y][x]         k < dim     (component-major)
+ *   tensor field w[k][c][z][y][x]         k < 3 (2D: xx, yy, xy)
+ *                                         k < 6 (3D: xx, yy, zz, xy, xz, yz)
+ *   sinograms    f[c][z][angle][bin]
+ *
+ * Each declaration names the reference interface it replaces
+ * (radon_tgv_primal_dual_2d.py = 2D:, radon_tgv_primal_dual_3d.py = 3D:).
+ */
+#ifndef GRAPTOR_AMD_H
+#define GRAPTOR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GR_OK = 0,
+    GR_ERR_INVALID = 1,     /* bad argument or shape */
+    GR_ERR_OOM = 2,         /* device allocation failed (maps to MemoryError) */
+    GR_ERR_HIP = 3,         /* any other HIP runtime failure */
+    GR_ERR_UNSUPPORTED = 4  /* valid request this build does not implement */
+};
+
+enum { GR_COUPLING_UNCORRELATED = 0, GR_COUPLING_FROBENIUS = 1, GR_COUPLING_NUCLEAR = 2 };
+enum { GR_DISC_KL = 0, GR_DISC_L2 = 1 };
+enum { GR_REG_TV = 0, GR_REG_TGV = 1 };
+enum { GR_NORMEST_REFERENCE = 0, GR_NORMEST_POWER = 1 };
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *gr_last_error(void);
+/* ABI revision of this library. */
+int gr_abi_version(void);
+
+/* Replaces radon_struct() 2D:503-535 / 3D:947-979.  Host only: angles
+ * (float64, radians, na of them) -> ofs_out[na][4] = (cos/dw, sin/dw,
+ * offset, 1/cos) in float32, with near-vertical rays snapped as 2D:518-522. */
+int gr_radon_struct(const double *angles, int na, int nx, int ny, int nd,
+                    double detector_width, double detector_shift, float *ofs_out);
+
+/* Device-side form of r_struct (2D:531-535): the angle table plus the
+ * projector's per-angle constants and work chunks.  ofs is a host table
+ * ofs[na][4] as produced by gr_radon_struct. */
+typedef struct gr_geometry gr_geometry;
+int gr_geometry_create(const float *ofs, int na, int nx, int ny, int nd,
+                       gr_geometry **out);
+int gr_geometry_destroy(gr_geometry *g);
+/* Tile shapes of the projector kernels (autotuner hook); 0 keeps a value. */
+int gr_geometry_set_tuning(gr_geometry *g, int fwd_band, int adj_tile_y);
+int gr_geometry_get_tuning(const gr_geometry *g, int *fwd_band, int *adj_tile_y);
+
+/* Replaces radon() 2D:544-550 / 3D:988-994 (kernel 2D:391-434):
+ * sino[s][angle][bin] = R img[s][y][x] for s < nslices. */
+int gr_radon(const gr_geometry *g, float *sino, const float *img, int nslices,
+             void *stream);
+
+/* Replaces radon_ad() 2D:559-565 / 3D:1002-1008 (kernel 2D:436-462):
+ * img[s] = R^T sino[s]. */
+int gr_radon_ad(const gr_geometry *g, float *img, const float *sino, int nslices,
+                void *stream);
+
+/* Replaces radon_normest() 2D:576-588 / 3D:1018-1030, computed in FP64 on
+ * the device over one nx*ny slice.  mode GR_NORMEST_REFERENCE: `iters`
+ * sum-normalised rounds from the host start image x0 (nx*ny doubles), as the
+ * reference; GR_NORMEST_POWER: L2-normalised power iteration from x0 (or a
+ * positive default when x0 is NULL) until the relative change is <= tol or
+ * `iters` rounds.  *out = estimate of ||R||_2.  Synchronises the stream. */
+int gr_normest(const gr_geometry *g, const double *x0, int iters, int mode,
+               double tol, double *out, void *stream);
+
+/* Replaces Make_function_feasible() 2D:476-486 / 3D:860-870: zero every bin
+ * of the device sinogram stack f (nslices slices) where R 1 <= 0. */
+int gr_make_feasible(const gr_geometry *g, float *f, int nslices, void *stream);
+
+/* Parameters of the iteration of tgv_radon() 2D:710-851 / 3D:1135-1292,
+ * already scaled the way 2D:752-762 / 3D:1180-1189 derive them. */
+typedef struct gr_tgv_params {
+    int dim;             /* 2 or 3 */
+    int nx, ny, nz, nch; /* grid per slice, slices per channel, channels */
+    int coupling;        /* GR_COUPLING_* */
+    int discrepancy;     /* GR_DISC_* */
+    int regularisation;  /* GR_REG_* */
+    float stepsize;      /* z spacing h (3D:631) */
+    float sigma, tau;    /* dual / primal steps */
+    float sigma_p;       /* sigma / L, the sinogram dual step */
+    float norming;       /* 1 / L */
+    float alpha_inv0;    /* 1 / (alpha0 / L^2) */
+    float alpha_inv1;    /* 1 / (alpha1 / L^2) */
+} gr_tgv_params;
+
+/* Caller-owned device state.  u, ubar: nch*nz*ny*nx; p, pbar, v: dim
+ * components; w: 3 (2D) or 6 (3D) components; lam, f, ktl: sinogram /
+ * image stacks; mfac: nch floats (mu for KL, mu/(sigma+mu) for L2); ku may
+ * be NULL unless ku_on_last or gr_tgv_energy needs it. */
+typedef struct gr_tgv_state {
+    float *u, *ubar, *p, *pbar, *v, *w;
+    float *lam, *ktl, *ku;
+    const float *f, *mfac;
+} gr_tgv_state;
+
+/* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 (dual step,
+ * projections, forward projection fused with the KL/L2 prox, backprojection,
+ * primal step fused with the extrapolation) on `stream`.  With ku_on_last,
+ * the last iteration also stores R(ubar) into ku, the sinogram the reference
+ * returns (2D:840-841). */
+int gr_tgv_iterate(const gr_geometry *g, const gr_tgv_params *prm,
+                   const gr_tgv_state *st, int n_iter, int ku_on_last,
+                   void *stream);
+
+/* gr_tgv_iterate with a HIP event pair around each of the four kernels of
+ * every iteration (measurement only; no counterpart in the reference).
+ * ms_out[0..3] receive the summed kernel times in milliseconds of the dual
+ * step, forward projection + prox, backprojection and primal step.
+ * Synchronises the stream. */
+int gr_tgv_profile(const gr_geometry *g, const gr_tgv_params *prm,
+                   const gr_tgv_state *st, int n_iter, double *ms_out, void *stream);
+
+/* Primal energy at (u, p) reduced in FP64 on the device (no counterpart in
+ * the reference's hot path; its Gap_TGV_KL 3D:882-927 is unused):
+ *   KL: sum_c mu_c KL(R u_c, f_c) + TGV_{alpha/L}(u; p)
+ *   L2: sum_c mu_c/2 |R u_c - f_c|^2 + TGV_alpha(u; p)
+ * out[0] = data term, out[1] = regulariser, out[2] = sum.  Uses st->ku as
+ * scratch for R u.  Synchronises the stream. */
+int gr_tgv_energy(const gr_geometry *g, const gr_tgv_params *prm,
+                  const gr_tgv_state *st, const double *mu, double alpha0,
+                  double alpha1, double L, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRAPTOR_AMD_H */
