@@ -28,7 +28,8 @@ EXPORTS = (
     "gr_last_error", "gr_abi_version", "gr_radon_struct", "gr_geometry_create",
     "gr_geometry_destroy", "gr_geometry_set_tuning", "gr_geometry_get_tuning",
     "gr_radon", "gr_radon_ad", "gr_normest", "gr_make_feasible", "gr_tgv_iterate",
-    "gr_tgv_energy", "gr_tgv_profile",
+    "gr_tgv_energy", "gr_tgv_profile", "gr_plan_create", "gr_plan_destroy",
+    "gr_plan_iterate", "gr_plan_iterate_graph", "gr_plan_profile", "gr_plan_retune",
 )
 
 
@@ -94,6 +95,13 @@ def load(path: str | None = None):
                                V, D, D, D, V, V], I),
             "gr_tgv_profile": ([V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState),
                                 I, V, V], I),
+            "gr_plan_create": ([I, V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState),
+                                ctypes.POINTER(V)], I),
+            "gr_plan_destroy": ([V], I),
+            "gr_plan_iterate": ([V, I, I, V], I),
+            "gr_plan_iterate_graph": ([V, I, I, V], I),
+            "gr_plan_profile": ([V, I, V, V], I),
+            "gr_plan_retune": ([V, V], I),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

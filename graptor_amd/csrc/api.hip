@@ -5,8 +5,11 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
+#include <tuple>
 #include <vector>
 
 struct gr_geometry {
@@ -36,8 +39,6 @@ int hip_status(hipError_t e, const char* where)
 using namespace gr;
 
 namespace {
-
-constexpr int kFwdThreads = 256;
 
 // Split each angle class into chunks of at most `cap` (angle, bin) items.
 std::vector<FwdChunk> make_chunks(const std::vector<FwdAngle>& sorted, int nd, int cap)
@@ -69,11 +70,7 @@ std::vector<FwdChunk> make_chunks(const std::vector<FwdAngle>& sorted, int nd, i
     return out;
 }
 
-int stream_of(void* s, hipStream_t* out)
-{
-    *out = reinterpret_cast<hipStream_t>(s);
-    return GR_OK;
-}
+hipStream_t stream_of(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 bool valid_params(const gr_geometry* g, const gr_tgv_params* p)
 {
@@ -84,8 +81,243 @@ bool valid_params(const gr_geometry* g, const gr_tgv_params* p)
     if (p->dim == 2 && p->nz != 1) return false;
     if (p->coupling < 0 || p->coupling > 2) return false;
     if (p->discrepancy < 0 || p->discrepancy > 1) return false;
+    if (p->regularisation < 0 || p->regularisation > 1) return false;
     if (p->stepsize <= 0.f) return false;
     return true;
+}
+
+bool valid_state(const gr_tgv_params* p, const gr_tgv_state* s, bool need_ku)
+{
+    const bool tgv = p->regularisation == GR_REG_TGV;
+    if (!s || !s->u || !s->ubar || !s->v || !s->lam || !s->f || !s->mfac) return false;
+    if (tgv && (!s->p || !s->pbar || !s->w)) return false;
+    if (need_ku && !s->ku) return false;
+    return true;
+}
+
+// Descriptor of one problem with the geometry's tuning applied for its
+// slice-group size K.
+ProbDev make_prob(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s)
+{
+    const Geometry& g = gg->g;
+    ProbDev P{};
+    P.dim = p->dim; P.nx = p->nx; P.ny = p->ny; P.nz = p->nz; P.nch = p->nch;
+    P.nd = g.nd; P.na = g.na;
+    P.coupling = p->coupling;
+    P.tgv = p->regularisation == GR_REG_TGV ? 1 : 0;
+    P.disc = p->discrepancy;
+    P.hinv = 1.f / p->stepsize;
+    P.sigma = p->sigma; P.tau = p->tau; P.sigma_p = p->sigma_p; P.norming = p->norming;
+    P.ainv0 = p->alpha_inv0; P.ainv1 = p->alpha_inv1;
+    const int K = slice_group(p->nz * p->nch);
+    P.fwd_pitch = fwd_pitch_of(g.nx, g.ny);
+    P.fwd_band = g.fwd_band > 0 ? g.fwd_band : fwd_band_auto(P.fwd_pitch, K);
+    P.adj_tile_y = g.adj_tile_y;
+    P.adj_segw = adj_segw_of(P.adj_tile_y);
+    P.adj_achunk = adj_achunk_of(g.na, P.adj_segw, K);
+    P.ofs = reinterpret_cast<const float4*>(g.ofs_dev);
+    P.fwd = g.fwd_dev;
+    P.u = s->u; P.ubar = s->ubar; P.p = s->p; P.pbar = s->pbar; P.v = s->v; P.w = s->w;
+    P.lam = s->lam; P.ktl = s->ktl; P.ku = s->ku; P.f = s->f; P.mfac = s->mfac;
+    return P;
+}
+
+int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_on_last,
+                   hipStream_t st, double* ms_out)
+{
+    const int nsl = P.nz * P.nch;
+    const int pmode = P.disc == GR_DISC_KL ? FWD_KL : FWD_L2;
+    hipEvent_t ev[4];
+    if (ms_out) {
+        for (int k = 0; k < 4; ++k) GR_HIP(hipEventCreate(&ev[k]));
+        for (int k = 0; k < 3; ++k) ms_out[k] = 0.0;
+    }
+    int rc = GR_OK;
+    for (int it = 0; it < n_iter && rc == GR_OK; ++it) {
+        if (ms_out) (void)hipEventRecord(ev[0], st);
+        rc = launch_dual(P, st);
+        if (ms_out) (void)hipEventRecord(ev[1], st);
+        const int mode = pmode | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
+        if (!rc) rc = launch_fwd(P, gg->g, nsl, mode, st);
+        if (ms_out) (void)hipEventRecord(ev[2], st);
+        if (!rc) rc = launch_adj(P, nsl, true, st);
+        if (ms_out) {
+            (void)hipEventRecord(ev[3], st);
+            hipError_t e = hipEventSynchronize(ev[3]);
+            if (e != hipSuccess && !rc) rc = hip_status(e, "tgv_profile");
+            for (int k = 0; k < 3 && !rc; ++k) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+                ms_out[k] += ms;
+            }
+        }
+    }
+    if (ms_out)
+        for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
+    return rc;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// batch plans
+// --------------------------------------------------------------------------
+struct gr_plan {
+    int nprob = 0;
+    std::vector<ProbDev> probs;
+    ProbDev* probs_dev = nullptr;
+    struct FwdLaunch { int K, mi; size_t lds; std::vector<FwdItem> items; FwdItem* dev; };
+    struct AdjLaunch { int K, dim, tgv; size_t lds; std::vector<AdjItem> items; AdjItem* dev; };
+    struct DualLaunch { int dim, coupling, tgv, maxc; std::vector<DualItem> items; DualItem* dev; };
+    std::vector<FwdLaunch> fwd;
+    std::vector<AdjLaunch> adj;
+    std::vector<DualLaunch> dual;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_block = 0;
+};
+
+namespace {
+
+void plan_free_device(gr_plan* pl)
+{
+    if (pl->probs_dev) (void)hipFree(pl->probs_dev);
+    pl->probs_dev = nullptr;
+    for (auto& l : pl->fwd) if (l.dev) (void)hipFree(l.dev);
+    for (auto& l : pl->adj) if (l.dev) (void)hipFree(l.dev);
+    for (auto& l : pl->dual) if (l.dev) (void)hipFree(l.dev);
+    if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
+    pl->graph = nullptr;
+    pl->graph_block = 0;
+}
+
+template <typename T>
+int upload(const std::vector<T>& v, T** out)
+{
+    *out = nullptr;
+    if (v.empty()) return GR_OK;
+    GR_HIP(hipMalloc(out, sizeof(T) * v.size()));
+    GR_HIP(hipMemcpy(*out, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return GR_OK;
+}
+
+int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
+{
+    plan_free_device(pl);
+    pl->fwd.clear();
+    pl->adj.clear();
+    pl->dual.clear();
+    const int n = pl->nprob;
+    // forward chunk table: fewest items per thread giving >= 1024 work-groups
+    int t = 0;
+    for (; t < 3; ++t) {
+        long long total = 0;
+        for (int i = 0; i < n; ++i) {
+            const Geometry& g = geoms[i]->g;
+            const int S = pl->probs[i].nz * pl->probs[i].nch;
+            const int K = slice_group(S);
+            total += (long long)((S + K - 1) / K) * (g.tab_off[t + 1] - g.tab_off[t]);
+        }
+        if (total >= 1024) break;
+    }
+    const int mi = 8 >> t;
+    std::map<std::tuple<int, int>, size_t> fwd_idx;
+    std::map<std::tuple<int, int, int>, size_t> adj_idx;
+    std::map<std::tuple<int, int, int, int>, size_t> dual_idx;
+    for (int i = 0; i < n; ++i) {
+        const ProbDev& P = pl->probs[i];
+        const Geometry& g = geoms[i]->g;
+        const int S = P.nz * P.nch;
+        const int K = slice_group(S);
+        // forward items: slice groups x chunks (group-major)
+        auto fk = std::make_tuple(K, mi);
+        if (!fwd_idx.count(fk)) {
+            fwd_idx[fk] = pl->fwd.size();
+            pl->fwd.push_back({K, mi, 0, {}, nullptr});
+        }
+        auto& fl = pl->fwd[fwd_idx[fk]];
+        fl.lds = std::max(fl.lds, fwd_lds_bytes(P, K));
+        for (int s0 = 0; s0 < S; s0 += K)
+            for (int c = g.tab_off[t]; c < g.tab_off[t + 1]; ++c) {
+                const FwdChunk& ch = g.chunks_host[c];
+                fl.items.push_back(FwdItem{i, ch.cls, ch.a0, ch.na, ch.b0, ch.nb, s0, 0});
+            }
+        // backprojection + primal items: tiles x slice groups
+        auto ak = std::make_tuple(K, P.dim, P.tgv);
+        if (!adj_idx.count(ak)) {
+            adj_idx[ak] = pl->adj.size();
+            pl->adj.push_back({K, P.dim, P.tgv, 0, {}, nullptr});
+        }
+        auto& al = pl->adj[adj_idx[ak]];
+        al.lds = std::max(al.lds, adj_lds_bytes(P, K));
+        for (int s0 = 0; s0 < S; s0 += K)
+            for (int y0 = 0; y0 < P.ny; y0 += P.adj_tile_y)
+                for (int x0 = 0; x0 < P.nx; x0 += ADJ_TX) al.items.push_back(AdjItem{i, x0, y0, s0});
+        // dual items: 64 x 4 tiles of every plane
+        const int maxc = P.nch <= 4 ? 4 : 8;
+        auto dk = std::make_tuple(P.dim, P.coupling, P.tgv, maxc);
+        if (!dual_idx.count(dk)) {
+            dual_idx[dk] = pl->dual.size();
+            pl->dual.push_back({P.dim, P.coupling, P.tgv, maxc, {}, nullptr});
+        }
+        auto& dl = pl->dual[dual_idx[dk]];
+        for (int z = 0; z < P.nz; ++z)
+            for (int y0 = 0; y0 < P.ny; y0 += DUAL_BY)
+                for (int x0 = 0; x0 < P.nx; x0 += DUAL_BX) dl.items.push_back(DualItem{i, x0, y0, z});
+    }
+    for (auto& l : pl->fwd)
+        if (l.lds > 160 * 1024) {
+            set_error("plan: forward LDS stage of %zu bytes too large", l.lds);
+            return GR_ERR_INVALID;
+        }
+    int rc = upload(pl->probs, &pl->probs_dev);
+    for (auto& l : pl->fwd) if (!rc) rc = upload(l.items, &l.dev);
+    for (auto& l : pl->adj) if (!rc) rc = upload(l.items, &l.dev);
+    for (auto& l : pl->dual) if (!rc) rc = upload(l.items, &l.dev);
+    return rc;
+}
+
+int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
+{
+    int rc = GR_OK;
+    if (phase == 0) {
+        for (auto& l : pl->dual)
+            if (!rc) rc = launch_dual_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim,
+                                            l.coupling, l.tgv != 0, l.maxc, st);
+    } else if (phase == 1) {
+        for (auto& l : pl->fwd)
+            if (!rc) rc = launch_fwd_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.mi,
+                                           l.lds, fwd_mode, st);
+    } else {
+        for (auto& l : pl->adj)
+            if (!rc) rc = launch_adj_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.dim,
+                                           l.tgv != 0, l.lds, st);
+    }
+    return rc;
+}
+
+int plan_iter(gr_plan* pl, int n_iter, int ku_on_last, hipStream_t st)
+{
+    for (int it = 0; it < n_iter; ++it) {
+        const int mode = FWD_PROX | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
+        int rc = plan_phase(pl, 0, mode, st);
+        if (!rc) rc = plan_phase(pl, 1, mode, st);
+        if (!rc) rc = plan_phase(pl, 2, mode, st);
+        if (rc) return rc;
+    }
+    return GR_OK;
+}
+
+ProbDev standalone(const gr_geometry* gg, int nslices)
+{
+    const Geometry& g = gg->g;
+    ProbDev P{};
+    P.dim = 2; P.nx = g.nx; P.ny = g.ny; P.nz = 1; P.nch = nslices; P.nd = g.nd; P.na = g.na;
+    P.ofs = reinterpret_cast<const float4*>(g.ofs_dev);
+    P.fwd = g.fwd_dev;
+    P.fwd_band = g.fwd_band;
+    P.adj_tile_y = g.adj_tile_y;
+    return P;
 }
 
 }  // namespace
@@ -138,6 +370,7 @@ int gr_geometry_create(const float* ofs, int na, int nx, int ny, int nd, gr_geom
     g.nx = nx; g.ny = ny; g.nd = nd; g.na = na;
     g.fwd_band = 0;
     g.adj_tile_y = 16;
+    g.ofs_dev = nullptr; g.fwd_dev = nullptr; g.chunks_dev = nullptr;
     std::vector<FwdAngle> fa(na);
     for (int j = 0; j < na; ++j) {
         const double c = ofs[4 * j], s = ofs[4 * j + 1], o = ofs[4 * j + 2];
@@ -159,25 +392,22 @@ int gr_geometry_create(const float* ofs, int na, int nx, int ny, int nd, gr_geom
     }
     std::stable_sort(fa.begin(), fa.end(),
                      [](const FwdAngle& x, const FwdAngle& y) { return x.cls < y.cls; });
-    std::vector<FwdChunk> all;
     const int ipt[4] = {8, 4, 2, 1};
     for (int t = 0; t < 4; ++t) {
-        g.tab_off[t] = (int)all.size();
-        auto c = make_chunks(fa, nd, kFwdThreads * ipt[t]);
-        all.insert(all.end(), c.begin(), c.end());
+        g.tab_off[t] = (int)g.chunks_host.size();
+        auto c = make_chunks(fa, nd, FWD_THREADS * ipt[t]);
+        g.chunks_host.insert(g.chunks_host.end(), c.begin(), c.end());
     }
-    g.tab_off[4] = (int)all.size();
-    g.ofs_host = (float*)std::malloc(sizeof(float) * 4 * na);
-    std::memcpy(g.ofs_host, ofs, sizeof(float) * 4 * na);
+    g.tab_off[4] = (int)g.chunks_host.size();
     hipError_t e = hipMalloc(&g.ofs_dev, sizeof(float) * 4 * na);
     if (e == hipSuccess) e = hipMalloc(&g.fwd_dev, sizeof(FwdAngle) * na);
-    if (e == hipSuccess) e = hipMalloc(&g.chunks_dev, sizeof(FwdChunk) * all.size());
+    if (e == hipSuccess) e = hipMalloc(&g.chunks_dev, sizeof(FwdChunk) * g.chunks_host.size());
     if (e == hipSuccess)
         e = hipMemcpy(g.ofs_dev, ofs, sizeof(float) * 4 * na, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(g.fwd_dev, fa.data(), sizeof(FwdAngle) * na, hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMemcpy(g.chunks_dev, all.data(), sizeof(FwdChunk) * all.size(),
+        e = hipMemcpy(g.chunks_dev, g.chunks_host.data(), sizeof(FwdChunk) * g.chunks_host.size(),
                       hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         int rc = hip_status(e, "geometry_create");
@@ -195,7 +425,6 @@ int gr_geometry_destroy(gr_geometry* gg)
     if (g.ofs_dev) (void)hipFree(g.ofs_dev);
     if (g.fwd_dev) (void)hipFree(g.fwd_dev);
     if (g.chunks_dev) (void)hipFree(g.chunks_dev);
-    std::free(g.ofs_host);
     delete gg;
     return GR_OK;
 }
@@ -207,7 +436,7 @@ int gr_geometry_set_tuning(gr_geometry* gg, int fwd_band, int adj_tile_y)
         return GR_ERR_INVALID;
     }
     if (fwd_band < 0 || fwd_band > 256 || adj_tile_y < 0 ||
-        (adj_tile_y && (adj_tile_y < 4 || adj_tile_y > 32 || (adj_tile_y & 3)))) {
+        (adj_tile_y && (adj_tile_y < 4 || adj_tile_y > 4 * ADJ_MAXR || (adj_tile_y & 3)))) {
         set_error("set_tuning: fwd_band in [1,256], adj_tile_y multiple of 4 in [4,32]");
         return GR_ERR_INVALID;
     }
@@ -233,10 +462,10 @@ int gr_radon(const gr_geometry* gg, float* sino, const float* img, int nslices, 
         set_error("radon: invalid arguments");
         return GR_ERR_INVALID;
     }
-    hipStream_t st;
-    stream_of(stream, &st);
-    return launch_radon_fwd(gg->g, img, sino, nullptr, nullptr, nullptr, nslices, 1,
-                                   FWD_WRITE, 0.f, st);
+    ProbDev P = standalone(gg, nslices);
+    P.ubar = const_cast<float*>(img);
+    P.ku = sino;
+    return launch_fwd(P, gg->g, nslices, FWD_WRITE, stream_of(stream));
 }
 
 int gr_radon_ad(const gr_geometry* gg, float* img, const float* sino, int nslices, void* stream)
@@ -245,9 +474,10 @@ int gr_radon_ad(const gr_geometry* gg, float* img, const float* sino, int nslice
         set_error("radon_ad: invalid arguments");
         return GR_ERR_INVALID;
     }
-    hipStream_t st;
-    stream_of(stream, &st);
-    return launch_radon_adj(gg->g, sino, img, nslices, st);
+    ProbDev P = standalone(gg, nslices);
+    P.lam = const_cast<float*>(sino);
+    P.ktl = img;
+    return launch_adj(P, nslices, false, stream_of(stream));
 }
 
 int gr_normest(const gr_geometry* gg, const double* x0, int iters, int mode, double tol,
@@ -258,9 +488,7 @@ int gr_normest(const gr_geometry* gg, const double* x0, int iters, int mode, dou
         set_error("normest: invalid arguments");
         return GR_ERR_INVALID;
     }
-    hipStream_t st;
-    stream_of(stream, &st);
-    return launch_normest(gg->g, x0, iters, mode, tol, out, st);
+    return launch_normest(gg->g, x0, iters, mode, tol, out, stream_of(stream));
 }
 
 int gr_make_feasible(const gr_geometry* gg, float* f, int nslices, void* stream)
@@ -269,94 +497,30 @@ int gr_make_feasible(const gr_geometry* gg, float* f, int nslices, void* stream)
         set_error("make_feasible: invalid arguments");
         return GR_ERR_INVALID;
     }
-    hipStream_t st;
-    stream_of(stream, &st);
-    return launch_feasible(gg->g, f, nslices, st);
+    return launch_feasible(gg->g, f, nslices, stream_of(stream));
 }
 
 int gr_tgv_iterate(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
                    int n_iter, int ku_on_last, void* stream)
 {
-    if (!valid_params(gg, p) || !s || n_iter < 0) {
-        set_error("tgv_iterate: invalid parameters");
+    if (!valid_params(gg, p) || n_iter < 0 || !valid_state(p, s, ku_on_last != 0)) {
+        set_error("tgv_iterate: invalid parameters or missing state buffer");
         return GR_ERR_INVALID;
     }
-    const bool tgv = p->regularisation == GR_REG_TGV;
-    if (!s->u || !s->ubar || !s->v || !s->lam || !s->ktl || !s->f || !s->mfac ||
-        (tgv && (!s->p || !s->pbar || !s->w)) || (ku_on_last && !s->ku)) {
-        set_error("tgv_iterate: missing state buffer");
-        return GR_ERR_INVALID;
-    }
-    hipStream_t st;
-    stream_of(stream, &st);
-    TgvDims d{p->dim, p->nx, p->ny, p->nz, p->nch, p->coupling, tgv ? 1 : 0, p->stepsize};
-    const int nsl = p->nz * p->nch;
-    const int pmode = p->discrepancy == GR_DISC_KL ? FWD_KL : FWD_L2;
-    for (int it = 0; it < n_iter; ++it) {
-        int rc = launch_dual(d, s->v, s->w, s->ubar, s->pbar, p->sigma, p->alpha_inv0,
-                             p->alpha_inv1, st);
-        if (rc) return rc;
-        const int mode = pmode | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
-        rc = launch_radon_fwd(gg->g, s->ubar, s->ku, s->lam, s->f, s->mfac, nsl, p->nz, mode,
-                                     p->sigma_p, st);
-        if (rc) return rc;
-        rc = launch_radon_adj(gg->g, s->lam, s->ktl, nsl, st);
-        if (rc) return rc;
-        rc = launch_primal(d, s->u, s->ubar, s->p, s->pbar, s->v, s->w, s->ktl, p->tau,
-                           p->norming, st);
-        if (rc) return rc;
-    }
-    return GR_OK;
+    if (n_iter == 0) return GR_OK;
+    const ProbDev P = make_prob(gg, p, s);
+    return iterate_single(gg, P, n_iter, ku_on_last, stream_of(stream), nullptr);
 }
 
 int gr_tgv_profile(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
                    int n_iter, double* ms_out, void* stream)
 {
-    if (!valid_params(gg, p) || !s || n_iter <= 0 || !ms_out) {
-        set_error("tgv_profile: invalid parameters");
+    if (!valid_params(gg, p) || n_iter <= 0 || !ms_out || !valid_state(p, s, false)) {
+        set_error("tgv_profile: invalid parameters or missing state buffer");
         return GR_ERR_INVALID;
     }
-    const bool tgv = p->regularisation == GR_REG_TGV;
-    if (!s->u || !s->ubar || !s->v || !s->lam || !s->ktl || !s->f || !s->mfac ||
-        (tgv && (!s->p || !s->pbar || !s->w))) {
-        set_error("tgv_profile: missing state buffer");
-        return GR_ERR_INVALID;
-    }
-    hipStream_t st;
-    stream_of(stream, &st);
-    TgvDims d{p->dim, p->nx, p->ny, p->nz, p->nch, p->coupling, tgv ? 1 : 0, p->stepsize};
-    const int nsl = p->nz * p->nch;
-    const int pmode = p->discrepancy == GR_DISC_KL ? FWD_KL : FWD_L2;
-    hipEvent_t ev[5];
-    for (int k = 0; k < 5; ++k) GR_HIP(hipEventCreate(&ev[k]));
-    double acc[4] = {0, 0, 0, 0};
-    int rc = GR_OK;
-    for (int it = 0; it < n_iter && rc == GR_OK; ++it) {
-        (void)hipEventRecord(ev[0], st);
-        rc = launch_dual(d, s->v, s->w, s->ubar, s->pbar, p->sigma, p->alpha_inv0,
-                         p->alpha_inv1, st);
-        (void)hipEventRecord(ev[1], st);
-        if (!rc)
-            rc = launch_radon_fwd(gg->g, s->ubar, s->ku, s->lam, s->f, s->mfac, nsl, p->nz,
-                                  pmode, p->sigma_p, st);
-        (void)hipEventRecord(ev[2], st);
-        if (!rc) rc = launch_radon_adj(gg->g, s->lam, s->ktl, nsl, st);
-        (void)hipEventRecord(ev[3], st);
-        if (!rc)
-            rc = launch_primal(d, s->u, s->ubar, s->p, s->pbar, s->v, s->w, s->ktl, p->tau,
-                               p->norming, st);
-        (void)hipEventRecord(ev[4], st);
-        hipError_t e = hipEventSynchronize(ev[4]);
-        if (e != hipSuccess && !rc) rc = hip_status(e, "tgv_profile");
-        for (int k = 0; k < 4 && !rc; ++k) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-            acc[k] += ms;
-        }
-    }
-    for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
-    for (int k = 0; k < 4; ++k) ms_out[k] = acc[k];
-    return rc;
+    const ProbDev P = make_prob(gg, p, s);
+    return iterate_single(gg, P, n_iter, 0, stream_of(stream), ms_out);
 }
 
 int gr_tgv_energy(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
@@ -372,19 +536,19 @@ int gr_tgv_energy(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
         set_error("tgv_energy: p required for TGV");
         return GR_ERR_INVALID;
     }
-    hipStream_t st;
-    stream_of(stream, &st);
+    hipStream_t st = stream_of(stream);
     const int nsl = p->nz * p->nch;
-    int rc = launch_radon_fwd(gg->g, s->u, s->ku, nullptr, nullptr, nullptr, nsl, p->nz,
-                                     FWD_WRITE, 0.f, st);
+    ProbDev P = make_prob(gg, p, s);
+    ProbDev Pf = P;
+    Pf.ubar = s->u;  // project u itself
+    int rc = launch_fwd(Pf, gg->g, nsl, FWD_WRITE, st);
     if (rc) return rc;
     double data = 0.0, reg = 0.0;
     const size_t per_ch = (size_t)gg->g.nd * gg->g.na * p->nz;
     rc = launch_data_energy(s->ku, s->f, mu, p->nch, per_ch, p->discrepancy, &data, st);
     if (rc) return rc;
     const double scale = p->discrepancy == GR_DISC_KL ? 1.0 / L : 1.0;
-    TgvDims d{p->dim, p->nx, p->ny, p->nz, p->nch, p->coupling, tgv ? 1 : 0, p->stepsize};
-    rc = launch_energy(d, s->u, s->p, alpha0 * scale, alpha1 * scale, &reg, st);
+    rc = launch_energy(P, alpha0 * scale, alpha1 * scale, &reg, st);
     if (rc) return rc;
     out[0] = data;
     out[1] = reg;
@@ -392,5 +556,141 @@ int gr_tgv_energy(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
     return GR_OK;
 }
 
-}  // extern "C"
+int gr_plan_create(int nprob, const gr_geometry* const* geoms, const gr_tgv_params* prms,
+                   const gr_tgv_state* states, gr_plan** out)
+{
+    if (nprob <= 0 || !geoms || !prms || !states || !out) {
+        set_error("plan_create: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    *out = nullptr;
+    for (int i = 0; i < nprob; ++i)
+        if (!valid_params(geoms[i], &prms[i]) || !valid_state(&prms[i], &states[i], false)) {
+            set_error("plan_create: problem %d has invalid parameters or state", i);
+            return GR_ERR_INVALID;
+        }
+    gr_plan* pl = new (std::nothrow) gr_plan();
+    if (!pl) {
+        set_error("plan_create: host allocation failed");
+        return GR_ERR_OOM;
+    }
+    pl->nprob = nprob;
+    for (int i = 0; i < nprob; ++i) pl->probs.push_back(make_prob(geoms[i], &prms[i], &states[i]));
+    int rc = plan_build(pl, geoms);
+    if (rc) {
+        gr_plan_destroy(pl);
+        return rc;
+    }
+    *out = pl;
+    return GR_OK;
+}
 
+int gr_plan_destroy(gr_plan* pl)
+{
+    if (!pl) return GR_OK;
+    plan_free_device(pl);
+    if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
+    delete pl;
+    return GR_OK;
+}
+
+int gr_plan_iterate(gr_plan* pl, int n_iter, int ku_on_last, void* stream)
+{
+    if (!pl || n_iter < 0) {
+        set_error("plan_iterate: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    if (ku_on_last)
+        for (const auto& P : pl->probs)
+            if (!P.ku) {
+                set_error("plan_iterate: ku_on_last needs every problem's ku buffer");
+                return GR_ERR_INVALID;
+            }
+    return plan_iter(pl, n_iter, ku_on_last, stream_of(stream));
+}
+
+int gr_plan_iterate_graph(gr_plan* pl, int n_iter, int block, void* stream)
+{
+    if (!pl || n_iter < 0 || block <= 0) {
+        set_error("plan_iterate_graph: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    hipStream_t st = stream_of(stream);
+    if (pl->graph && pl->graph_block != block) {
+        (void)hipGraphExecDestroy(pl->graph);
+        pl->graph = nullptr;
+    }
+    if (!pl->graph && n_iter >= block) {
+        if (!pl->cap_stream)
+            GR_HIP(hipStreamCreateWithFlags(&pl->cap_stream, hipStreamNonBlocking));
+        GR_HIP(hipStreamBeginCapture(pl->cap_stream, hipStreamCaptureModeRelaxed));
+        int rc = plan_iter(pl, block, 0, pl->cap_stream);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(pl->cap_stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_status(e, "plan_iterate_graph capture");
+        e = hipGraphInstantiate(&pl->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            pl->graph = nullptr;
+            return hip_status(e, "plan_iterate_graph instantiate");
+        }
+        pl->graph_block = block;
+    }
+    int done = 0;
+    if (pl->graph)
+        for (; done + block <= n_iter; done += block) GR_HIP(hipGraphLaunch(pl->graph, st));
+    return plan_iter(pl, n_iter - done, 0, st);
+}
+
+int gr_plan_profile(gr_plan* pl, int n_iter, double* ms_out, void* stream)
+{
+    if (!pl || n_iter <= 0 || !ms_out) {
+        set_error("plan_profile: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    hipStream_t st = stream_of(stream);
+    hipEvent_t ev[4];
+    for (int k = 0; k < 4; ++k) GR_HIP(hipEventCreate(&ev[k]));
+    for (int k = 0; k < 3; ++k) ms_out[k] = 0.0;
+    int rc = GR_OK;
+    for (int it = 0; it < n_iter && !rc; ++it) {
+        for (int ph = 0; ph < 3 && !rc; ++ph) {
+            (void)hipEventRecord(ev[ph], st);
+            rc = plan_phase(pl, ph, FWD_PROX, st);
+        }
+        (void)hipEventRecord(ev[3], st);
+        hipError_t e = hipEventSynchronize(ev[3]);
+        if (e != hipSuccess && !rc) rc = hip_status(e, "plan_profile");
+        for (int k = 0; k < 3 && !rc; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+            ms_out[k] += ms;
+        }
+    }
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
+    return rc;
+}
+
+int gr_plan_retune(gr_plan* pl, const gr_geometry* const* geoms)
+{
+    if (!pl || !geoms) {
+        set_error("plan_retune: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    for (int i = 0; i < pl->nprob; ++i) {
+        ProbDev& P = pl->probs[i];
+        const Geometry& g = geoms[i]->g;
+        const int K = slice_group(P.nz * P.nch);
+        P.fwd_band = g.fwd_band > 0 ? g.fwd_band : fwd_band_auto(P.fwd_pitch, K);
+        P.adj_tile_y = g.adj_tile_y;
+        P.adj_segw = adj_segw_of(P.adj_tile_y);
+        P.adj_achunk = adj_achunk_of(g.na, P.adj_segw, K);
+    }
+    return plan_build(pl, geoms);
+}
+
+}  // extern "C"

@@ -1,8 +1,10 @@
 // Shared declarations of the graptor_amd HIP library (gfx950 / CDNA4).
 #pragma once
 #include <hip/hip_runtime.h>
-#include <stdint.h>
 #include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
 
 #include "../../include/graptor_amd.h"
 
@@ -46,6 +48,27 @@ struct FwdChunk {
     int cls, a0, na, b0, nb, pad0, pad1, pad2;
 };
 
+// Everything a kernel needs to know about one reconstruction problem (or one
+// standalone projector call): sizes, derived scalars, geometry tables and
+// device state.  Passed by value for single problems and through a device
+// table for batches.
+struct ProbDev {
+    int dim, nx, ny, nz, nch, nd, na;
+    int coupling, tgv, disc;
+    float hinv, sigma, tau, sigma_p, norming, ainv0, ainv1;
+    int fwd_band, fwd_pitch, adj_tile_y, adj_segw, adj_achunk;
+    const float4* ofs;     // na x (cos, sin, off, 1/cos)
+    const FwdAngle* fwd;   // na, sorted by class
+    // state; for standalone projector calls: ubar = image in, ku = sinogram
+    // out (forward), lam = sinogram in, ktl = image out (backprojection)
+    float *u, *ubar, *p, *pbar, *v, *w, *lam, *ktl, *ku;
+    const float *f, *mfac;
+};
+
+struct FwdItem { int prob, cls, a0, na, b0, nb, s0, pad; };
+struct AdjItem { int prob, x0, y0, s0; };
+struct DualItem { int prob, x0, y0, z; };
+
 // Device-side geometry (owned by gr_geometry).
 struct Geometry {
     int nx, ny, nd, na;
@@ -54,39 +77,52 @@ struct Geometry {
     FwdChunk* chunks_dev;    // four chunk tables back to back
     int tab_off[5];          // table t = chunks [tab_off[t], tab_off[t+1]),
                              // built for 8, 4, 2, 1 items per thread
-    float* ofs_host;         // copy of the table (host)
-    int fwd_band;            // lines per LDS band in the forward kernel
+    std::vector<FwdChunk> chunks_host;
+    int fwd_band;            // lines per LDS band (0: automatic)
     int adj_tile_y;          // rows per backprojection tile
 };
 
-// Launch helpers implemented in projector.hip.
-int launch_radon_fwd(const Geometry& g, const float* img, float* sino,
-                     float* lam, const float* f, const float* mfac, int nslices,
-                     int nz_per_ch, int mode, float sigp, hipStream_t st);
-int launch_radon_adj(const Geometry& g, const float* sino, float* img, int nslices,
-                     hipStream_t st);
+// Modes of the forward projector epilogue.
+// FWD_PROX applies each problem's own discrepancy (batches).
+enum { FWD_WRITE = 1, FWD_KL = 2, FWD_L2 = 4, FWD_PROX = 8 };
+
+constexpr int FWD_THREADS = 256;
+constexpr int FWD_PADL = 4;          // zero candidates left of position 0
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_TX = 64;           // backprojection tile width (one wave)
+constexpr int ADJ_MAXR = 8;          // rows per thread (tile_y <= 32)
+constexpr int DUAL_BX = 64, DUAL_BY = 4;
+
+// Kernel-side sizes derived from a problem and its tuning.
+int fwd_pitch_of(int nx, int ny);
+int fwd_band_auto(int pitch, int K);
+int adj_segw_of(int tile_y);
+int adj_achunk_of(int na, int segw, int K);
+size_t fwd_lds_bytes(const ProbDev& P, int K);
+size_t adj_lds_bytes(const ProbDev& P, int K);
+int slice_group(int nslices);  // K = 4, 2 or 1 slices sharing geometry
+
+// ---- launchers (single problem: descriptor by value) ---------------------
+int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipStream_t st);
+// adjoint only (primal == false: writes K^T lam into P.ktl) or fused with
+// the primal step of the iteration (primal == true)
+int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st);
+int launch_dual(const ProbDev& P, hipStream_t st);
+
+// ---- launchers (batches: descriptor table + item table) -------------------
+int launch_fwd_batch(const ProbDev* probs_dev, const FwdItem* items, int nitems, int K,
+                     int mi, size_t lds, int mode, hipStream_t st);
+int launch_adj_batch(const ProbDev* probs_dev, const AdjItem* items, int nitems, int K,
+                     int dim, bool tgv, size_t lds, hipStream_t st);
+int launch_dual_batch(const ProbDev* probs_dev, const DualItem* items, int nitems, int dim,
+                      int coupling, bool tgv, int maxc, hipStream_t st);
+
+// ---- auxiliary (aux.hip) ---------------------------------------------------
 int launch_normest(const Geometry& g, const double* x0_host, int iters, int mode,
                    double tol, double* out, hipStream_t st);
 int launch_feasible(const Geometry& g, float* f, int nslices, hipStream_t st);
-
-// Modes of the fused forward projector epilogue.
-enum { FWD_WRITE = 1, FWD_KL = 2, FWD_L2 = 4 };
-
-// TGV stencil launchers implemented in tgv.hip.
-struct TgvDims {
-    int dim, nx, ny, nz, nch;
-    int coupling, tgv;
-    float h;
-};
-int launch_dual(const TgvDims& d, float* v, float* w, const float* ubar,
-                const float* pbar, float sigma, float ainv0, float ainv1,
-                hipStream_t st);
-int launch_primal(const TgvDims& d, float* u, float* ubar, float* p, float* pbar,
-                  const float* v, const float* w, const float* ktl, float tau,
-                  float norming, hipStream_t st);
-int launch_energy(const TgvDims& d, const float* u, const float* p, double a0,
-                  double a1, double* out_reg, hipStream_t st);
-int launch_data_energy(const float* ku, const float* f, const double* mu,
-                       int nch, size_t per_ch, int disc, double* out, hipStream_t st);
+int launch_energy(const ProbDev& P, double a0, double a1, double* out_reg, hipStream_t st);
+int launch_data_energy(const float* ku, const float* f, const double* mu, int nch,
+                       size_t per_ch, int disc, double* out, hipStream_t st);
 
 }  // namespace gr

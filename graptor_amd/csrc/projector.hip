@@ -1,7 +1,10 @@
-// Parallel-beam projector pair for gfx950: forward projection (ray gather
-// through LDS bands, fused with the KL/L2 sinogram prox) and backprojection
-// (pixel tiles gathering from LDS-staged sinogram segments), plus the FP64
-// operator-norm power iteration and the feasibility mask.
+// Parallel-beam projector pair for gfx950.
+//
+// Forward projection: ray gather through LDS bands, fused with the KL/L2
+// sinogram prox.  Backprojection: pixel tiles gathering from LDS-staged
+// sinogram segments, optionally fused with the primal step of the
+// primal-dual iteration (u, p updates and their extrapolations), so that
+// K^T lam never goes through HBM.
 //
 // Discretisation (radon_tgv_primal_dual_2d.py 2D:391-462): with detector
 // coordinate t(x, y) = x*cos + y*sin + off, bin i of the forward projection
@@ -12,18 +15,10 @@
 
 #include <algorithm>
 #include <cmath>
-#include <vector>
 
 namespace gr {
 
 namespace {
-
-constexpr int FWD_THREADS = 256;
-constexpr int FWD_MAX_ITEMS = 8;     // (angle, bin) items per thread
-constexpr int FWD_PADL = 4;          // zero candidates left of position 0
-constexpr int ADJ_THREADS = 256;
-constexpr int ADJ_TX = 64;           // tile width = one wavefront of columns
-constexpr int ADJ_MAXR = 8;          // rows per thread (tile_y <= 32)
 
 // Map hardware block order (dealt round-robin over the 8 XCDs) to a work
 // order in which consecutive work ids run on the same XCD and share its L2.
@@ -69,40 +64,36 @@ __device__ __forceinline__ void lds_store(float* sh, int pos, const float (&v)[K
 }
 
 // --------------------------------------------------------------------------
-// Forward projection.  A work-group owns K slices (sharing all geometry)
-// and one chunk of (angle, bin) items whose angles all step along the same
-// axis.  It streams the K slices through LDS in bands of `band` lines of
-// that axis; per line every item reads the <= 3 candidates on the other
-// axis whose hat weight can be positive.  After the last band each item
-// holds its complete ray sum and applies the sinogram epilogue.
+// Forward projection of one work item: K slices s0.. (sharing all geometry)
+// times a chunk of (angle, bin) items whose angles step along the same axis.
+// The K slices stream through LDS in bands of `band` lines of that axis; per
+// line every item reads the <= 3 candidates on the other axis whose hat
+// weight can be positive.  After the last band each item holds its complete
+// ray sum and applies the sinogram epilogue.
 // --------------------------------------------------------------------------
-template <int K>
-__global__ __launch_bounds__(FWD_THREADS) void radon_fwd_kernel(
-    const float* __restrict__ img, float* __restrict__ sino,
-    float* __restrict__ lam, const float* __restrict__ f,
-    const float* __restrict__ mfac, const FwdAngle* __restrict__ angs,
-    const FwdChunk* __restrict__ chunks, int nchunks, int nx, int ny, int nd,
-    int na, int nslices, int nz_per_ch, int band, int pitch, int mode, float sigp)
+template <int K, int MI>
+__device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int mode,
+                                         float* sh)
 {
-    extern __shared__ float sh[];
-    const int work = xcd_work_id(blockIdx.x, gridDim.x);
-    const int chunk_id = work % nchunks;
-    const int s0 = (work / nchunks) * K;
-    const FwdChunk ch = chunks[chunk_id];
+    if (mode & FWD_PROX) mode |= P.disc == GR_DISC_KL ? FWD_KL : FWD_L2;
     const int tid = threadIdx.x;
-    const int len_step = ch.cls == 0 ? nx : ny;
-    const int len_cand = ch.cls == 0 ? ny : nx;
-    const int nitems = ch.na * ch.nb;
+    const int nx = P.nx, ny = P.ny, nd = P.nd, na = P.na;
+    const int nslices = P.nz * P.nch;
+    const int band = P.fwd_band, pitch = P.fwd_pitch;
+    const int len_step = I.cls == 0 ? nx : ny;
+    const int len_cand = I.cls == 0 ? ny : nx;
+    const int nitems = I.na * I.nb;
+    const int s0 = I.s0;
+    const float* __restrict__ img = P.ubar;
 
-    float acc[FWD_MAX_ITEMS][K];
-    float fi[FWD_MAX_ITEMS], ms[FWD_MAX_ITEMS], mc[FWD_MAX_ITEMS];
-    float of[FWD_MAX_ITEMS], iv[FWD_MAX_ITEMS], hh[FWD_MAX_ITEMS];
+    float acc[MI][K];
+    float fi[MI], ms[MI], mc[MI], of[MI], iv[MI], hh[MI];
 #pragma unroll
-    for (int it = 0; it < FWD_MAX_ITEMS; ++it) {
+    for (int it = 0; it < MI; ++it) {
         const int item = tid + it * FWD_THREADS;
-        const int a = item < nitems ? item / ch.nb : 0;
-        const FwdAngle A = angs[ch.a0 + a];
-        fi[it] = (float)(ch.b0 + (item < nitems ? item - a * ch.nb : 0));
+        const int a = item < nitems ? item / I.nb : 0;
+        const FwdAngle A = P.fwd[I.a0 + a];
+        fi[it] = (float)(I.b0 + (item < nitems ? item - a * I.nb : 0));
         ms[it] = A.mstep; mc[it] = A.mcand; of[it] = A.off;
         iv[it] = A.invc;  hh[it] = A.hc;
 #pragma unroll
@@ -117,7 +108,7 @@ __global__ __launch_bounds__(FWD_THREADS) void radon_fwd_kernel(
     for (int l0 = 0; l0 < len_step; l0 += band) {
         __syncthreads();
         // stage lines [l0, l0+band) of K slices, candidates contiguous
-        if (ch.cls == 1) {  // lines are rows, candidates are x
+        if (I.cls == 1) {  // lines are rows, candidates are x
             for (int e = tid; e < band * len_cand; e += FWD_THREADS) {
                 const int li = e / len_cand, x = e - li * len_cand;
                 const int y = l0 + li;
@@ -149,7 +140,7 @@ __global__ __launch_bounds__(FWD_THREADS) void radon_fwd_kernel(
         __syncthreads();
 
 #pragma unroll
-        for (int it = 0; it < FWD_MAX_ITEMS; ++it) {
+        for (int it = 0; it < MI; ++it) {
             if (tid + it * FWD_THREADS >= nitems) continue;
             for (int st = 0; st < band; ++st) {
                 const float xs = (float)(l0 + st);
@@ -178,60 +169,153 @@ __global__ __launch_bounds__(FWD_THREADS) void radon_fwd_kernel(
 
     // epilogue: store and/or apply the sinogram prox (2D:316-338)
 #pragma unroll
-    for (int it = 0; it < FWD_MAX_ITEMS; ++it) {
+    for (int it = 0; it < MI; ++it) {
         const int item = tid + it * FWD_THREADS;
         if (item >= nitems) continue;
-        const int a = item / ch.nb;
-        const int i = ch.b0 + item - a * ch.nb;
-        const int id = angs[ch.a0 + a].id;
+        const int a = item / I.nb;
+        const int i = I.b0 + item - a * I.nb;
+        const int id = P.fwd[I.a0 + a].id;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int s = s0 + k;
             if (s >= nslices) continue;
             const size_t idx = ((size_t)s * na + id) * nd + i;
             const float val = acc[it][k];
-            if (mode & FWD_WRITE) sino[idx] = val;
+            if (mode & FWD_WRITE) P.ku[idx] = val;
             if (mode & FWD_KL) {
-                const float m = mfac[s / nz_per_ch];
-                const float sv = lam[idx] + sigp * val;
+                const float m = P.mfac[s / P.nz];
+                const float sv = P.lam[idx] + P.sigma_p * val;
                 const float z = sv - m;
-                const float d = z * z + 4.f * sigp * f[idx] * m;
-                lam[idx] = sv - 0.5f * (z + sqrtf(d));
+                const float d = z * z + 4.f * P.sigma_p * P.f[idx] * m;
+                P.lam[idx] = sv - 0.5f * (z + sqrtf(d));
             } else if (mode & FWD_L2) {
-                const float m = mfac[s / nz_per_ch];
-                lam[idx] = (lam[idx] + sigp * (val - f[idx])) * m;
+                const float m = P.mfac[s / P.nz];
+                P.lam[idx] = (P.lam[idx] + P.sigma_p * (val - P.f[idx])) * m;
             }
         }
     }
 }
 
-// --------------------------------------------------------------------------
-// Backprojection.  A work-group owns a 64 x tile_y pixel tile of K slices.
-// Per chunk of angles it stages, for each angle, only the sinogram segment
-// the tile projects onto (bins outside [0, nd) staged as zeros, which
-// reproduces the reference's edge guards), then every thread interpolates
-// for its tile_y/4 pixels.
-// --------------------------------------------------------------------------
-template <int K>
-__global__ __launch_bounds__(ADJ_THREADS) void radon_adj_kernel(
-    const float* __restrict__ sino, float* __restrict__ img,
-    const float4* __restrict__ ofs, int nx, int ny, int nd, int na, int nslices,
-    int tile_y, int segw, int achunk, int tiles_x, int ntiles)
+template <int K, int MI>
+__global__ __launch_bounds__(FWD_THREADS) void fwd_single_kernel(ProbDev P,
+                                                                 const FwdChunk* __restrict__ chunks,
+                                                                 int nchunks, int mode)
 {
     extern __shared__ float sh[];
+    const int work = xcd_work_id(blockIdx.x, gridDim.x);
+    const FwdChunk ch = chunks[work % nchunks];
+    const FwdItem I{0, ch.cls, ch.a0, ch.na, ch.b0, ch.nb, (work / nchunks) * K, 0};
+    fwd_body<K, MI>(P, I, mode, sh);
+}
+
+template <int K, int MI>
+__global__ __launch_bounds__(FWD_THREADS) void fwd_batch_kernel(const ProbDev* __restrict__ probs,
+                                                                const FwdItem* __restrict__ items,
+                                                                int mode)
+{
+    extern __shared__ float sh[];
+    const FwdItem I = items[xcd_work_id(blockIdx.x, gridDim.x)];
+    const ProbDev P = probs[I.prob];
+    fwd_body<K, MI>(P, I, mode, sh);
+}
+
+// --------------------------------------------------------------------------
+// Primal step at one voxel-channel (update_u 2D:340-357 / 3D:706-729,
+// update_p 2D:359-376 / 3D:734-757, update_extra 2D:637-638):
+//   u+ = max(0, u + tau (div v - ktl / L)),  ubar = 2 u+ - u
+//   p+ = p + tau (v + div2 w),               pbar = 2 p+ - p
+// i indexes u[c][z][y][x] (= slice s = z + nz c), z = s % nz.
+// --------------------------------------------------------------------------
+template <int DIM, bool TGV>
+__device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x, int y, int z,
+                                              float ktl)
+{
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const size_t plane = (size_t)nx * ny;
+    const size_t cs = plane * nz * P.nch;  // component stride
+    const float* __restrict__ v = P.v;
+    float vc[3];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
+    float d0 = (x == nx - 1) ? 0.f : vc[0];
+    if (x > 0) d0 -= v[i - 1];
+    float d1 = (y == ny - 1) ? 0.f : vc[1];
+    if (y > 0) d1 -= v[i + cs - nx];
+    float dv = d0 + d1;
+    if constexpr (DIM == 3) {
+        float d2 = (z == nz - 1) ? 0.f : vc[2];
+        if (z > 0) d2 -= v[i + 2 * cs - plane];
+        dv += d2 * P.hinv;
+    }
+    const float uo = P.u[i];
+    float un = uo + P.tau * (dv - P.norming * ktl);
+    un = un < 0.f ? 0.f : un;
+    P.u[i] = un;
+    P.ubar[i] = 2.f * un - uo;
+
+    if constexpr (TGV) {
+        const float* __restrict__ w = P.w;
+        constexpr int M = DIM == 2 ? 3 : 6;
+        float wc[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) wc[m] = w[i + m * cs];
+        float q[3];
+        if constexpr (DIM == 2) {
+            // x-part uses (w_xx, w_xy), y-part (w_xy, w_yy)
+            float x0 = (x == 0) ? 0.f : -wc[0], x1 = (x == 0) ? 0.f : -wc[2];
+            if (x < nx - 1) { x0 += w[i + 1]; x1 += w[i + 2 * cs + 1]; }
+            float y0 = (y == 0) ? 0.f : -wc[2], y1 = (y == 0) ? 0.f : -wc[1];
+            if (y < ny - 1) { y0 += w[i + 2 * cs + nx]; y1 += w[i + cs + nx]; }
+            q[0] = x0 + y0;
+            q[1] = x1 + y1;
+        } else {
+            // tensor order (xx, yy, zz, xy, xz, yz): x-part (0,3,4), y-part (3,1,5),
+            // z-part (4,5,2)
+            const int xi[3] = {0, 3, 4}, yi[3] = {3, 1, 5}, zi[3] = {4, 5, 2};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                float xs = (x == 0) ? 0.f : -wc[xi[k]];
+                if (x < nx - 1) xs += w[i + xi[k] * cs + 1];
+                float ys = (y == 0) ? 0.f : -wc[yi[k]];
+                if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
+                float zs = (z == 0) ? 0.f : -wc[zi[k]];
+                if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
+                q[k] = xs + ys + zs * P.hinv;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) {
+            const float po = P.p[i + k * cs];
+            const float pn = po + P.tau * (vc[k] + q[k]);
+            P.p[i + k * cs] = pn;
+            P.pbar[i + k * cs] = 2.f * pn - po;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Backprojection of one tile: 64 x tile_y pixels of K slices s0...  Per
+// chunk of angles it stages, for each angle, only the sinogram segment the
+// tile projects onto (bins outside [0, nd) staged as zeros, which reproduces
+// the reference's edge guards), then every thread interpolates for its
+// tile_y/4 pixels.  PRIMAL fuses the primal step into the epilogue.
+// --------------------------------------------------------------------------
+template <int K, bool PRIMAL, int DIM, bool TGV>
+__device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, float* sh)
+{
+    const int nx = P.nx, ny = P.ny, nd = P.nd, na = P.na;
+    const int nslices = P.nz * P.nch;
+    const int tile_y = P.adj_tile_y, segw = P.adj_segw, achunk = P.adj_achunk;
     float* seg = sh;                                  // achunk * segw * K
     float4* par = reinterpret_cast<float4*>(sh + (size_t)achunk * segw * K);
     int* lo_s = reinterpret_cast<int*>(par + achunk);
 
-    const int work = xcd_work_id(blockIdx.x, gridDim.x);
-    const int tile = work % ntiles;
-    const int s0 = (work / ntiles) * K;
-    const int x0 = (tile % tiles_x) * ADJ_TX;
-    const int y0 = (tile / tiles_x) * tile_y;
+    const int x0 = I.x0, y0 = I.y0, s0 = I.s0;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int R = tile_y >> 2;
     const float xf = (float)(x0 + lane);
+    const float* __restrict__ sino = P.lam;
 
     float acc[ADJ_MAXR][K];
 #pragma unroll
@@ -244,7 +328,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void radon_adj_kernel(
         const int A = min(achunk, na - a0);
         __syncthreads();
         if (tid < A) {
-            const float4 o = ofs[a0 + tid];
+            const float4 o = P.ofs[a0 + tid];
             const float tmin = o.z + (float)x0 * o.x + (float)y0 * o.y +
                                fminf(0.f, (float)(ADJ_TX - 1) * o.x) +
                                fminf(0.f, (float)(tile_y - 1) * o.y);
@@ -290,6 +374,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void radon_adj_kernel(
 
     const int x = x0 + lane;
     if (x >= nx) return;
+    const size_t plane = (size_t)nx * ny;
 #pragma unroll
     for (int r = 0; r < ADJ_MAXR; ++r) {
         if (r >= R) break;
@@ -298,167 +383,57 @@ __global__ __launch_bounds__(ADJ_THREADS) void radon_adj_kernel(
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int s = s0 + k;
-            if (s < nslices) img[(size_t)s * nx * ny + (size_t)y * nx + x] = acc[r][k];
+            if (s >= nslices) continue;
+            const size_t i = (size_t)s * plane + (size_t)y * nx + x;
+            if constexpr (PRIMAL)
+                primal_update<DIM, TGV>(P, i, x, y, s % P.nz, acc[r][k]);
+            else
+                P.ktl[i] = acc[r][k];
         }
     }
 }
 
-// --------------------------------------------------------------------------
-// FP64 kernels for the operator-norm power iteration (one slice).
-// --------------------------------------------------------------------------
-__global__ void fwd64_kernel(const double* __restrict__ img, double* __restrict__ sino,
-                             const FwdAngle* __restrict__ angs, int nx, int ny, int nd,
-                             int na)
+template <int K, bool PRIMAL, int DIM, bool TGV>
+__global__ __launch_bounds__(ADJ_THREADS) void adj_single_kernel(ProbDev P, int tiles_x,
+                                                                 int ntiles)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= nd * na) return;
-    const int aa = gid / nd, i = gid - aa * nd;
-    const FwdAngle A = angs[aa];
-    const int len_step = A.cls == 0 ? nx : ny;
-    const int len_cand = A.cls == 0 ? ny : nx;
-    const double ms = A.mstep, mc = A.mcand, off = A.off;
-    double acc = 0.0;
-    for (int l = 0; l < len_step; ++l) {
-        const double r = (double)l * ms + off - (double)i;
-        // four candidates tolerate any rounding of the float constants
-        const double q = -r / mc;
-        const int b = (int)ceil(q - 1.0 / fabs(mc)) - 1;
-        for (int k = 0; k < 4; ++k) {
-            const int c = b + k;
-            if (c < 0 || c >= len_cand) continue;
-            const double wgt = 1.0 - fabs((double)c * mc + r);
-            if (wgt <= 0.0) continue;
-            const int x = A.cls == 0 ? l : c;
-            const int y = A.cls == 0 ? c : l;
-            acc += wgt * img[(size_t)y * nx + x];
-        }
-    }
-    sino[(size_t)A.id * nd + i] = acc;
+    extern __shared__ float sh[];
+    const int work = xcd_work_id(blockIdx.x, gridDim.x);
+    const int tile = work % ntiles;
+    const AdjItem I{0, (tile % tiles_x) * ADJ_TX, (tile / tiles_x) * P.adj_tile_y,
+                    (work / ntiles) * K};
+    adj_body<K, PRIMAL, DIM, TGV>(P, I, sh);
 }
 
-__global__ void adj64_kernel(const double* __restrict__ sino, double* __restrict__ img,
-                             const float4* __restrict__ ofs, int nx, int ny, int nd, int na)
+template <int K, int DIM, bool TGV>
+__global__ __launch_bounds__(ADJ_THREADS) void adj_batch_kernel(const ProbDev* __restrict__ probs,
+                                                                const AdjItem* __restrict__ items)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= nx * ny) return;
-    const int y = gid / nx, x = gid - y * nx;
-    double acc = 0.0;
-    for (int j = 0; j < na; ++j) {
-        const float4 o = ofs[j];
-        const double t = (double)x * o.x + (double)y * o.y + (double)o.z;
-        if (t > -1.0 && t < (double)nd) {
-            const double fl = floor(t);
-            const double p = t - fl;
-            const int i0 = (int)fl;
-            if (i0 >= 0) acc += (1.0 - p) * sino[(size_t)j * nd + i0];
-            if (i0 <= nd - 2) acc += p * sino[(size_t)j * nd + i0 + 1];
-        }
-    }
-    img[gid] = acc;
-}
-
-// Single-block FP64 reduction: mode 0 sum(x), 1 sum(x*x), 2 sum(x*y).
-__global__ __launch_bounds__(1024) void reduce64_kernel(const double* __restrict__ x,
-                                                        const double* __restrict__ y,
-                                                        int n, int mode, double* out)
-{
-    __shared__ double red[1024];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) {
-        const double a = x[i];
-        acc += mode == 0 ? a : (mode == 1 ? a * a : a * y[i]);
-    }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = red[0];
-}
-
-// x /= (*n) (mode 0) or x /= sqrt(*n) (mode 1)
-__global__ void scale64_kernel(double* x, int n, const double* nrm, int mode)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double d = mode == 0 ? *nrm : sqrt(*nrm);
-    x[i] = x[i] / d;
-}
-
-__global__ void feasible_kernel(float* f, const float* r1, size_t per_slice,
-                                size_t total)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    if (r1[i % per_slice] <= 0.f) f[i] = 0.f;
-}
-
-__global__ void fill_kernel(float* x, size_t n, float v)
-{
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = v;
+    extern __shared__ float sh[];
+    const AdjItem I = items[xcd_work_id(blockIdx.x, gridDim.x)];
+    const ProbDev P = probs[I.prob];
+    adj_body<K, true, DIM, TGV>(P, I, sh);
 }
 
 }  // namespace
 
 // --------------------------------------------------------------------------
-// host launchers
+// sizes
 // --------------------------------------------------------------------------
-static int fwd_pitch(const Geometry& g)
+int fwd_pitch_of(int nx, int ny)
 {
-    int p = std::max(g.nx, g.ny) + FWD_PADL + 4;
+    int p = std::max(nx, ny) + FWD_PADL + 4;
     if ((p & 1) == 0) ++p;  // odd pitch: conflict-free transposed stores
     return p;
 }
 
-template <int K>
-static int fwd_launch_k(const Geometry& g, const float* img, float* sino, float* lam,
-                        const float* f, const float* mfac, int nslices, int nz_per_ch,
-                        int mode, float sigp, hipStream_t st)
+int fwd_band_auto(int pitch, int K)
 {
-    const int pitch = fwd_pitch(g);
-    const int groups = (nslices + K - 1) / K;
-    // chunk table: fewest items per thread that still gives >= 1024 groups
-    int t = 0;
-    while (t < 3 && (long long)groups * (g.tab_off[t + 1] - g.tab_off[t]) < 1024) ++t;
-    const int nchunks = g.tab_off[t + 1] - g.tab_off[t];
-    const FwdChunk* chunks = g.chunks_dev + g.tab_off[t];
-    int band = g.fwd_band;
-    if (band <= 0) {  // largest multiple of 4 keeping the LDS stage near 50 KB
-        band = (int)(50 * 1024 / ((size_t)pitch * K * sizeof(float)));
-        band = std::max(4, std::min(32, band & ~3));
-    }
-    const size_t lds = (size_t)band * pitch * K * sizeof(float);
-    if (band <= 0 || lds > 160 * 1024) {
-        set_error("radon: band %d does not fit LDS (%zu bytes)", band, lds);
-        return GR_ERR_INVALID;
-    }
-    const long long blocks = (long long)groups * nchunks;
-    if (blocks <= 0 || blocks > 0x7fffffff) {
-        set_error("radon: invalid grid (%lld blocks)", blocks);
-        return GR_ERR_INVALID;
-    }
-    hipLaunchKernelGGL(radon_fwd_kernel<K>, dim3((unsigned)blocks), dim3(FWD_THREADS), lds,
-                       st, img, sino, lam, f, mfac, g.fwd_dev, chunks, nchunks, g.nx, g.ny,
-                       g.nd, g.na, nslices, nz_per_ch, band, pitch, mode, sigp);
-    GR_LAUNCH_CHECK("radon_fwd_kernel");
-    return GR_OK;
+    int b = (int)(50 * 1024 / ((size_t)pitch * K * sizeof(float)));
+    return std::max(4, std::min(32, b & ~3));
 }
 
-int launch_radon_fwd(const Geometry& g, const float* img, float* sino, float* lam,
-                     const float* f, const float* mfac, int nslices, int nz_per_ch,
-                     int mode, float sigp, hipStream_t st)
-{
-    if (nslices <= 0) return GR_OK;
-    if (nslices >= 4)
-        return fwd_launch_k<4>(g, img, sino, lam, f, mfac, nslices, nz_per_ch, mode, sigp, st);
-    if (nslices >= 2)
-        return fwd_launch_k<2>(g, img, sino, lam, f, mfac, nslices, nz_per_ch, mode, sigp, st);
-    return fwd_launch_k<1>(g, img, sino, lam, f, mfac, nslices, nz_per_ch, mode, sigp, st);
-}
-
-static int adj_segw(int tile_y)
+int adj_segw_of(int tile_y)
 {
     const double diag = std::sqrt((double)(ADJ_TX - 1) * (ADJ_TX - 1) +
                                   (double)(tile_y - 1) * (tile_y - 1));
@@ -466,149 +441,172 @@ static int adj_segw(int tile_y)
     return (w + 3) & ~3;
 }
 
+int adj_achunk_of(int na, int segw, int K)
+{
+    const size_t per_angle = (size_t)segw * K * sizeof(float) + sizeof(float4) + sizeof(int);
+    int a = (int)std::min<size_t>((size_t)na, (76 * 1024) / per_angle);
+    return std::max(a, 1);
+}
+
+size_t fwd_lds_bytes(const ProbDev& P, int K)
+{
+    return (size_t)P.fwd_band * P.fwd_pitch * K * sizeof(float);
+}
+
+size_t adj_lds_bytes(const ProbDev& P, int K)
+{
+    return (size_t)P.adj_achunk * ((size_t)P.adj_segw * K * sizeof(float) + sizeof(float4) +
+                                   sizeof(int));
+}
+
+int slice_group(int nslices) { return nslices >= 4 ? 4 : (nslices >= 2 ? 2 : 1); }
+
+// --------------------------------------------------------------------------
+// single-problem launchers
+// --------------------------------------------------------------------------
+template <int K, int MI>
+static void fwd_single_launch(const ProbDev& P, const FwdChunk* chunks, int nchunks, int groups,
+                              size_t lds, int mode, hipStream_t st)
+{
+    hipLaunchKernelGGL((fwd_single_kernel<K, MI>), dim3((unsigned)(groups * nchunks)),
+                       dim3(FWD_THREADS), lds, st, P, chunks, nchunks, mode);
+}
+
 template <int K>
-static int adj_launch_k(const Geometry& g, const float* sino, float* img, int nslices,
+static int fwd_single_k(const ProbDev& Pin, const Geometry& g, int nslices, int mode,
                         hipStream_t st)
 {
-    const int tile_y = g.adj_tile_y;
-    const int segw = adj_segw(tile_y);
-    const size_t per_angle = (size_t)segw * K * sizeof(float) + sizeof(float4) + sizeof(int);
-    int achunk = (int)std::min<size_t>((size_t)g.na, (76 * 1024) / per_angle);
-    achunk = std::max(achunk, 1);
-    const size_t lds = (size_t)achunk * per_angle;
-    const int tiles_x = (g.nx + ADJ_TX - 1) / ADJ_TX;
-    const int tiles_y = (g.ny + tile_y - 1) / tile_y;
-    const int ntiles = tiles_x * tiles_y;
+    ProbDev P = Pin;
     const int groups = (nslices + K - 1) / K;
-    const long long blocks = (long long)groups * ntiles;
-    if (blocks > 0x7fffffff) {
-        set_error("radon_ad: too many slices");
+    // chunk table: fewest items per thread that still gives >= 1024 groups
+    int t = 0;
+    while (t < 3 && (long long)groups * (g.tab_off[t + 1] - g.tab_off[t]) < 1024) ++t;
+    const int nchunks = g.tab_off[t + 1] - g.tab_off[t];
+    const FwdChunk* chunks = g.chunks_dev + g.tab_off[t];
+    P.fwd_pitch = fwd_pitch_of(P.nx, P.ny);
+    if (P.fwd_band <= 0) P.fwd_band = fwd_band_auto(P.fwd_pitch, K);
+    const size_t lds = fwd_lds_bytes(P, K);
+    if (lds > 160 * 1024) {
+        set_error("radon: band %d does not fit LDS (%zu bytes)", P.fwd_band, lds);
         return GR_ERR_INVALID;
     }
-    hipLaunchKernelGGL(radon_adj_kernel<K>, dim3((unsigned)blocks), dim3(ADJ_THREADS), lds, st,
-                       sino, img, reinterpret_cast<const float4*>(g.ofs_dev), g.nx, g.ny,
-                       g.nd, g.na, nslices, tile_y, segw, achunk, tiles_x, ntiles);
-    GR_LAUNCH_CHECK("radon_adj_kernel");
+    const long long blocks = (long long)groups * nchunks;
+    if (blocks <= 0 || blocks > 0x7fffffff) {
+        set_error("radon: invalid grid (%lld blocks)", blocks);
+        return GR_ERR_INVALID;
+    }
+    switch (t) {
+    case 0: fwd_single_launch<K, 8>(P, chunks, nchunks, groups, lds, mode, st); break;
+    case 1: fwd_single_launch<K, 4>(P, chunks, nchunks, groups, lds, mode, st); break;
+    case 2: fwd_single_launch<K, 2>(P, chunks, nchunks, groups, lds, mode, st); break;
+    default: fwd_single_launch<K, 1>(P, chunks, nchunks, groups, lds, mode, st); break;
+    }
+    GR_LAUNCH_CHECK("fwd_single_kernel");
     return GR_OK;
 }
 
-int launch_radon_adj(const Geometry& g, const float* sino, float* img, int nslices,
-                     hipStream_t st)
+int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipStream_t st)
 {
     if (nslices <= 0) return GR_OK;
-    if (g.adj_tile_y < 4 || g.adj_tile_y > 4 * ADJ_MAXR || (g.adj_tile_y & 3)) {
-        set_error("radon_ad: tile_y %d must be a multiple of 4 in [4, %d]", g.adj_tile_y,
+    switch (slice_group(nslices)) {
+    case 4: return fwd_single_k<4>(P, g, nslices, mode, st);
+    case 2: return fwd_single_k<2>(P, g, nslices, mode, st);
+    default: return fwd_single_k<1>(P, g, nslices, mode, st);
+    }
+}
+
+template <int K, bool PRIMAL, int DIM, bool TGV>
+static void adj_single_launch(const ProbDev& P, int blocks, int tiles_x, int ntiles, size_t lds,
+                              hipStream_t st)
+{
+    hipLaunchKernelGGL((adj_single_kernel<K, PRIMAL, DIM, TGV>), dim3((unsigned)blocks),
+                       dim3(ADJ_THREADS), lds, st, P, tiles_x, ntiles);
+}
+
+template <int K>
+static int adj_single_k(const ProbDev& Pin, int nslices, bool primal, hipStream_t st)
+{
+    ProbDev P = Pin;
+    if (P.adj_tile_y < 4 || P.adj_tile_y > 4 * ADJ_MAXR || (P.adj_tile_y & 3)) {
+        set_error("radon_ad: tile_y %d must be a multiple of 4 in [4, %d]", P.adj_tile_y,
                   4 * ADJ_MAXR);
         return GR_ERR_INVALID;
     }
-    if (nslices >= 4) return adj_launch_k<4>(g, sino, img, nslices, st);
-    if (nslices >= 2) return adj_launch_k<2>(g, sino, img, nslices, st);
-    return adj_launch_k<1>(g, sino, img, nslices, st);
-}
-
-int launch_normest(const Geometry& g, const double* x0_host, int iters, int mode,
-                   double tol, double* out, hipStream_t st)
-{
-    const int npx = g.nx * g.ny, nsb = g.nd * g.na;
-    double *x = nullptr, *s = nullptr, *y = nullptr, *red = nullptr;
-    auto cleanup = [&]() {
-        if (x) (void)hipFree(x);
-        if (s) (void)hipFree(s);
-        if (y) (void)hipFree(y);
-        if (red) (void)hipFree(red);
-    };
-    if (hipMalloc(&x, sizeof(double) * npx) != hipSuccess ||
-        hipMalloc(&y, sizeof(double) * npx) != hipSuccess ||
-        hipMalloc(&s, sizeof(double) * nsb) != hipSuccess ||
-        hipMalloc(&red, sizeof(double) * 2) != hipSuccess) {
-        cleanup();
-        set_error("normest: device allocation failed");
-        return GR_ERR_OOM;
+    P.adj_segw = adj_segw_of(P.adj_tile_y);
+    P.adj_achunk = adj_achunk_of(P.na, P.adj_segw, K);
+    const size_t lds = adj_lds_bytes(P, K);
+    const int tiles_x = (P.nx + ADJ_TX - 1) / ADJ_TX;
+    const int tiles_y = (P.ny + P.adj_tile_y - 1) / P.adj_tile_y;
+    const int ntiles = tiles_x * tiles_y;
+    const int groups = (nslices + K - 1) / K;
+    const long long blocks = (long long)groups * ntiles;
+    if (blocks <= 0 || blocks > 0x7fffffff) {
+        set_error("radon_ad: invalid grid (%lld blocks)", blocks);
+        return GR_ERR_INVALID;
     }
-    std::vector<double> h0(npx);
-    if (x0_host) {
-        std::copy(x0_host, x0_host + npx, h0.begin());
+    const int b = (int)blocks;
+    if (!primal) {
+        adj_single_launch<K, false, 2, false>(P, b, tiles_x, ntiles, lds, st);
+    } else if (P.dim == 2) {
+        if (P.tgv) adj_single_launch<K, true, 2, true>(P, b, tiles_x, ntiles, lds, st);
+        else adj_single_launch<K, true, 2, false>(P, b, tiles_x, ntiles, lds, st);
     } else {
-        for (int i = 0; i < npx; ++i) h0[i] = 1.0 + 0.25 * std::sin(0.37 * i);
+        if (P.tgv) adj_single_launch<K, true, 3, true>(P, b, tiles_x, ntiles, lds, st);
+        else adj_single_launch<K, true, 3, false>(P, b, tiles_x, ntiles, lds, st);
     }
-    int rc = GR_OK;
-    auto chk = [&](hipError_t e, const char* w) {
-        if (e != hipSuccess && rc == GR_OK) rc = hip_status(e, w);
-    };
-    chk(hipMemcpyAsync(x, h0.data(), sizeof(double) * npx, hipMemcpyHostToDevice, st),
-        "normest copy");
-    const int tb = 256;
-    double result = 0.0;
-    if (mode == GR_NORMEST_REFERENCE) {
-        for (int it = 0; it < iters && rc == GR_OK; ++it) {
-            hipLaunchKernelGGL(reduce64_kernel, dim3(1), dim3(1024), 0, st, x, x, npx, 0, red);
-            hipLaunchKernelGGL(scale64_kernel, dim3((npx + tb - 1) / tb), dim3(tb), 0, st, x,
-                               npx, red, 0);
-            hipLaunchKernelGGL(fwd64_kernel, dim3((nsb + tb - 1) / tb), dim3(tb), 0, st, x, s,
-                               g.fwd_dev, g.nx, g.ny, g.nd, g.na);
-            hipLaunchKernelGGL(adj64_kernel, dim3((npx + tb - 1) / tb), dim3(tb), 0, st, s, x,
-                               reinterpret_cast<const float4*>(g.ofs_dev), g.nx, g.ny, g.nd,
-                               g.na);
-            chk(hipGetLastError(), "normest launch");
-        }
-        double n = 0.0;
-        chk(hipMemcpyAsync(&n, red, sizeof(double), hipMemcpyDeviceToHost, st), "normest read");
-        chk(hipStreamSynchronize(st), "normest sync");
-        result = std::sqrt(n);
-    } else {
-        double prev = 0.0;
-        for (int it = 0; it < iters && rc == GR_OK; ++it) {
-            hipLaunchKernelGGL(reduce64_kernel, dim3(1), dim3(1024), 0, st, x, x, npx, 1, red);
-            hipLaunchKernelGGL(scale64_kernel, dim3((npx + tb - 1) / tb), dim3(tb), 0, st, x,
-                               npx, red, 1);
-            hipLaunchKernelGGL(fwd64_kernel, dim3((nsb + tb - 1) / tb), dim3(tb), 0, st, x, s,
-                               g.fwd_dev, g.nx, g.ny, g.nd, g.na);
-            hipLaunchKernelGGL(adj64_kernel, dim3((npx + tb - 1) / tb), dim3(tb), 0, st, s, y,
-                               reinterpret_cast<const float4*>(g.ofs_dev), g.nx, g.ny, g.nd,
-                               g.na);
-            hipLaunchKernelGGL(reduce64_kernel, dim3(1), dim3(1024), 0, st, x, y, npx, 2,
-                               red + 1);
-            chk(hipGetLastError(), "normest launch");
-            double lam = 0.0;
-            chk(hipMemcpyAsync(&lam, red + 1, sizeof(double), hipMemcpyDeviceToHost, st),
-                "normest read");
-            chk(hipStreamSynchronize(st), "normest sync");
-            std::swap(x, y);
-            result = std::sqrt(std::max(lam, 0.0));
-            if (it > 0 && std::fabs(lam - prev) <= tol * std::fabs(lam)) break;
-            prev = lam;
-        }
-    }
-    cleanup();
-    if (rc != GR_OK) return rc;
-    *out = result;
+    GR_LAUNCH_CHECK("adj_single_kernel");
     return GR_OK;
 }
 
-int launch_feasible(const Geometry& g, float* f, int nslices, hipStream_t st)
+int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st)
 {
-    const size_t npx = (size_t)g.nx * g.ny, per = (size_t)g.nd * g.na;
-    float *ones = nullptr, *r1 = nullptr;
-    if (hipMalloc(&ones, npx * 4) != hipSuccess || hipMalloc(&r1, per * 4) != hipSuccess) {
-        if (ones) (void)hipFree(ones);
-        set_error("make_feasible: device allocation failed");
-        return GR_ERR_OOM;
+    if (nslices <= 0) return GR_OK;
+    switch (slice_group(nslices)) {
+    case 4: return adj_single_k<4>(P, nslices, primal, st);
+    case 2: return adj_single_k<2>(P, nslices, primal, st);
+    default: return adj_single_k<1>(P, nslices, primal, st);
     }
-    const int tb = 256;
-    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((npx + tb - 1) / tb)), dim3(tb), 0, st, ones,
-                       npx, 1.0f);
-    int rc = launch_radon_fwd(g, ones, r1, nullptr, nullptr, nullptr, 1, 1, FWD_WRITE, 0.f, st);
-    if (rc == GR_OK) {
-        const size_t total = per * (size_t)nslices;
-        hipLaunchKernelGGL(feasible_kernel, dim3((unsigned)((total + tb - 1) / tb)), dim3(tb), 0,
-                           st, f, r1, per, total);
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = hip_status(e, "make_feasible");
+}
+
+// --------------------------------------------------------------------------
+// batch launchers
+// --------------------------------------------------------------------------
+int launch_fwd_batch(const ProbDev* probs, const FwdItem* items, int nitems, int K, int mi,
+                     size_t lds, int mode, hipStream_t st)
+{
+    if (nitems <= 0) return GR_OK;
+    dim3 grd((unsigned)nitems), blk(FWD_THREADS);
+#define GR_FB(KK, MM) \
+    hipLaunchKernelGGL((fwd_batch_kernel<KK, MM>), grd, blk, lds, st, probs, items, mode)
+#define GR_FB_K(KK)                                  \
+    switch (mi) {                                    \
+    case 8: GR_FB(KK, 8); break;                     \
+    case 4: GR_FB(KK, 4); break;                     \
+    case 2: GR_FB(KK, 2); break;                     \
+    default: GR_FB(KK, 1); break;                    \
     }
-    (void)hipFree(ones);
-    (void)hipFree(r1);
-    return rc;
+    if (K == 4) { GR_FB_K(4) } else if (K == 2) { GR_FB_K(2) } else { GR_FB_K(1) }
+#undef GR_FB_K
+#undef GR_FB
+    GR_LAUNCH_CHECK("fwd_batch_kernel");
+    return GR_OK;
+}
+
+int launch_adj_batch(const ProbDev* probs, const AdjItem* items, int nitems, int K, int dim,
+                     bool tgv, size_t lds, hipStream_t st)
+{
+    if (nitems <= 0) return GR_OK;
+    dim3 grd((unsigned)nitems), blk(ADJ_THREADS);
+#define GR_AB(KK, DD, TT) \
+    hipLaunchKernelGGL((adj_batch_kernel<KK, DD, TT>), grd, blk, lds, st, probs, items)
+#define GR_AB_K(KK)                                          \
+    if (dim == 2) { if (tgv) GR_AB(KK, 2, true); else GR_AB(KK, 2, false); } \
+    else { if (tgv) GR_AB(KK, 3, true); else GR_AB(KK, 3, false); }
+    if (K == 4) { GR_AB_K(4) } else if (K == 2) { GR_AB_K(2) } else { GR_AB_K(1) }
+#undef GR_AB_K
+#undef GR_AB
+    GR_LAUNCH_CHECK("adj_batch_kernel");
+    return GR_OK;
 }
 
 }  // namespace gr

@@ -1,14 +1,11 @@
 // TGV stencil kernels for gfx950.
 //
-// dual_kernel   : one thread per voxel, all channels: v += sigma (grad ubar - pbar),
-//                 w += sigma E(pbar), then the coupled projections of v and w
-//                 (update_v/update_w/update_NormV_*/update_NormW_* of
-//                 radon_tgv_primal_dual_2d.py 2D:98-304 and _3d.py 3D:290-660
-//                 fused into one pass).
-// primal_kernel : one thread per voxel-channel: u+ = max(0, u + tau (div v -
-//                 K^T lam / L)), p+ = p + tau (v + div2 w) and the
-//                 extrapolations ubar = 2u+ - u, pbar = 2p+ - p
-//                 (update_u/update_p/update_extra 2D:340-376, 2D:637-638).
+// dual step    : one thread per voxel, all channels: v += sigma (grad ubar - pbar),
+//                w += sigma E(pbar), then the coupled projections of v and w
+//                (update_v/update_w/update_NormV_*/update_NormW_* of
+//                radon_tgv_primal_dual_2d.py 2D:98-304 and _3d.py 3D:290-660
+//                fused into one pass).  The primal step lives in projector.hip,
+//                fused with the backprojection.
 // energy kernels: FP64 block partial sums of the primal energy.
 #include "common.h"
 
@@ -19,7 +16,6 @@ namespace gr {
 
 namespace {
 
-constexpr int BX = 64, BY = 4;
 constexpr float W_OFF = 1.4142f;  // the reference's float literal for sqrt(2)
 
 // ---- closed-form projection helpers -------------------------------------
@@ -105,16 +101,19 @@ __device__ __forceinline__ void jacobi3(const T (&A)[6], T (&ev)[3], T (&V)[3][3
 // ---- dual step ------------------------------------------------------------
 
 template <int DIM, int CPL, bool TGV, int MAXC>
-__global__ __launch_bounds__(BX* BY) void dual_kernel(
-    float* __restrict__ v, float* __restrict__ w, const float* __restrict__ ubar,
-    const float* __restrict__ pbar, int nx, int ny, int nz, int nch, float hinv,
-    float sigma, float ainv0, float ainv1)
+__device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
 {
     constexpr int D = DIM;
     constexpr int M = DIM == 2 ? 3 : 6;
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y * BY + threadIdx.y;
-    const int z = blockIdx.z;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nch = P.nch;
+    const float hinv = P.hinv, sigma = P.sigma, ainv0 = P.ainv0, ainv1 = P.ainv1;
+    float* __restrict__ v = P.v;
+    float* __restrict__ w = P.w;
+    const float* __restrict__ ubar = P.ubar;
+    const float* __restrict__ pbar = P.pbar;
+    const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
+    const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
+    const int z = I.z;
     if (x >= nx || y >= ny) return;
     const size_t plane = (size_t)nx * ny;
     const size_t chs = plane * nz;        // channel stride
@@ -276,79 +275,23 @@ __global__ __launch_bounds__(BX* BY) void dual_kernel(
     }
 }
 
-// ---- primal step + extrapolation ----------------------------------------
-
-template <int DIM, bool TGV>
-__global__ __launch_bounds__(BX* BY) void primal_kernel(
-    float* __restrict__ u, float* __restrict__ ubar, float* __restrict__ p,
-    float* __restrict__ pbar, const float* __restrict__ v, const float* __restrict__ w,
-    const float* __restrict__ ktl, int nx, int ny, int nz, int nch, float hinv, float tau,
-    float norming)
+template <int DIM, int CPL, bool TGV, int MAXC>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_single_kernel(ProbDev P, int tiles_x,
+                                                                       int tiles_y)
 {
-    const int x = blockIdx.x * BX + threadIdx.x;
-    const int y = blockIdx.y * BY + threadIdx.y;
-    const int zc = blockIdx.z;  // z + nz * channel
-    if (x >= nx || y >= ny) return;
-    const int z = zc % nz;
-    const size_t plane = (size_t)nx * ny;
-    const size_t cs = plane * nz * nch;
-    const size_t i = (size_t)zc * plane + (size_t)y * nx + x;
+    const int b = blockIdx.x;
+    const int tx = b % tiles_x, rest = b / tiles_x;
+    const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
+    dual_body<DIM, CPL, TGV, MAXC>(P, I);
+}
 
-    float vc[3];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
-    float d0 = (x == nx - 1) ? 0.f : vc[0];
-    if (x > 0) d0 -= v[i - 1];
-    float d1 = (y == ny - 1) ? 0.f : vc[1];
-    if (y > 0) d1 -= v[i + cs - nx];
-    float dv = d0 + d1;
-    if constexpr (DIM == 3) {
-        float d2 = (z == nz - 1) ? 0.f : vc[2];
-        if (z > 0) d2 -= v[i + 2 * cs - plane];
-        dv += d2 * hinv;
-    }
-    const float uo = u[i];
-    float un = uo + tau * (dv - norming * ktl[i]);
-    un = un < 0.f ? 0.f : un;
-    u[i] = un;
-    ubar[i] = 2.f * un - uo;
-
-    if constexpr (TGV) {
-        constexpr int M = DIM == 2 ? 3 : 6;
-        float wc[M];
-#pragma unroll
-        for (int m = 0; m < M; ++m) wc[m] = w[i + m * cs];
-        float q[3];
-        if constexpr (DIM == 2) {
-            // x-part uses (w_xx, w_xy), y-part (w_xy, w_yy)
-            float x0 = (x == 0) ? 0.f : -wc[0], x1 = (x == 0) ? 0.f : -wc[2];
-            if (x < nx - 1) { x0 += w[i + 1]; x1 += w[i + 2 * cs + 1]; }
-            float y0 = (y == 0) ? 0.f : -wc[2], y1 = (y == 0) ? 0.f : -wc[1];
-            if (y < ny - 1) { y0 += w[i + 2 * cs + nx]; y1 += w[i + cs + nx]; }
-            q[0] = x0 + y0;
-            q[1] = x1 + y1;
-        } else {
-            // tensor order (xx, yy, zz, xy, xz, yz): x-part (0,3,4), y-part (3,1,5), z-part (4,5,2)
-            const int xi[3] = {0, 3, 4}, yi[3] = {3, 1, 5}, zi[3] = {4, 5, 2};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                float xs = (x == 0) ? 0.f : -wc[xi[k]];
-                if (x < nx - 1) xs += w[i + xi[k] * cs + 1];
-                float ys = (y == 0) ? 0.f : -wc[yi[k]];
-                if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
-                float zs = (z == 0) ? 0.f : -wc[zi[k]];
-                if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
-                q[k] = xs + ys + zs * hinv;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-            const float po = p[i + k * cs];
-            const float pn = po + tau * (vc[k] + q[k]);
-            p[i + k * cs] = pn;
-            pbar[i + k * cs] = 2.f * pn - po;
-        }
-    }
+template <int DIM, int CPL, bool TGV, int MAXC>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_batch_kernel(
+    const ProbDev* __restrict__ probs, const DualItem* __restrict__ items)
+{
+    const DualItem I = items[blockIdx.x];
+    const ProbDev P = probs[I.prob];
+    dual_body<DIM, CPL, TGV, MAXC>(P, I);
 }
 
 // ---- FP64 energy ----------------------------------------------------------
@@ -490,92 +433,99 @@ __global__ __launch_bounds__(EN_THREADS) void data_energy_kernel(
 }  // namespace
 
 // --------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------
+template <int DIM, int CPL, bool TGV, int MC>
+static void dual_single_launch(const ProbDev& P, hipStream_t st)
+{
+    const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
+    const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
+    const long long blocks = (long long)tiles_x * tiles_y * P.nz;
+    hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC>), dim3((unsigned)blocks),
+                       dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+}
 
 template <int DIM, int CPL, bool TGV>
-static int dual_dispatch_c(const TgvDims& d, float* v, float* w, const float* ubar,
-                           const float* pbar, float sigma, float ainv0, float ainv1,
-                           hipStream_t st)
+static void dual_single_c(const ProbDev& P, hipStream_t st)
 {
-    dim3 blk(BX, BY), grd((d.nx + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz);
-    const float hinv = 1.f / d.h;
-#define GR_DUAL(MC)                                                                        \
-    hipLaunchKernelGGL((dual_kernel<DIM, CPL, TGV, MC>), grd, blk, 0, st, v, w, ubar, pbar, \
-                       d.nx, d.ny, d.nz, d.nch, hinv, sigma, ainv0, ainv1)
-    if (d.nch <= 1) GR_DUAL(1);
-    else if (d.nch <= 2) GR_DUAL(2);
-    else if (d.nch <= 4) GR_DUAL(4);
-    else GR_DUAL(8);
-#undef GR_DUAL
-    GR_LAUNCH_CHECK("dual_kernel");
-    return GR_OK;
+    if (P.nch <= 1) dual_single_launch<DIM, CPL, TGV, 1>(P, st);
+    else if (P.nch <= 2) dual_single_launch<DIM, CPL, TGV, 2>(P, st);
+    else if (P.nch <= 4) dual_single_launch<DIM, CPL, TGV, 4>(P, st);
+    else dual_single_launch<DIM, CPL, TGV, 8>(P, st);
 }
 
 template <int DIM, int CPL>
-static int dual_dispatch_t(const TgvDims& d, float* v, float* w, const float* ubar,
-                           const float* pbar, float sigma, float ainv0, float ainv1,
-                           hipStream_t st)
+static void dual_single_t(const ProbDev& P, hipStream_t st)
 {
-    if (d.tgv) return dual_dispatch_c<DIM, CPL, true>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
-    return dual_dispatch_c<DIM, CPL, false>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    if (P.tgv) dual_single_c<DIM, CPL, true>(P, st);
+    else dual_single_c<DIM, CPL, false>(P, st);
 }
 
 template <int DIM>
-static int dual_dispatch_d(const TgvDims& d, float* v, float* w, const float* ubar,
-                           const float* pbar, float sigma, float ainv0, float ainv1,
-                           hipStream_t st)
+static void dual_single_d(const ProbDev& P, hipStream_t st)
 {
-    switch (d.coupling) {
-    case GR_COUPLING_UNCORRELATED:
-        return dual_dispatch_t<DIM, GR_COUPLING_UNCORRELATED>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
-    case GR_COUPLING_FROBENIUS:
-        return dual_dispatch_t<DIM, GR_COUPLING_FROBENIUS>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
-    case GR_COUPLING_NUCLEAR:
-        return dual_dispatch_t<DIM, GR_COUPLING_NUCLEAR>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
-    }
-    set_error("unknown coupling %d", d.coupling);
-    return GR_ERR_INVALID;
+    if (P.coupling == GR_COUPLING_UNCORRELATED) dual_single_t<DIM, GR_COUPLING_UNCORRELATED>(P, st);
+    else if (P.coupling == GR_COUPLING_FROBENIUS) dual_single_t<DIM, GR_COUPLING_FROBENIUS>(P, st);
+    else dual_single_t<DIM, GR_COUPLING_NUCLEAR>(P, st);
 }
 
-int launch_dual(const TgvDims& d, float* v, float* w, const float* ubar, const float* pbar,
-                float sigma, float ainv0, float ainv1, hipStream_t st)
+int launch_dual(const ProbDev& P, hipStream_t st)
 {
-    if (d.nch > 8) {
-        set_error("dual step: %d channels exceed the 8 this build holds per voxel", d.nch);
+    if (P.nch > 8) {
+        set_error("dual step: %d channels exceed the 8 this build holds per voxel", P.nch);
         return GR_ERR_UNSUPPORTED;
     }
-    if (d.dim == 2) return dual_dispatch_d<2>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
-    return dual_dispatch_d<3>(d, v, w, ubar, pbar, sigma, ainv0, ainv1, st);
+    const long long blocks = (long long)((P.nx + DUAL_BX - 1) / DUAL_BX) *
+                             ((P.ny + DUAL_BY - 1) / DUAL_BY) * P.nz;
+    if (blocks > 0x7fffffff) {
+        set_error("dual step: grid too large");
+        return GR_ERR_INVALID;
+    }
+    if (P.dim == 2) dual_single_d<2>(P, st);
+    else dual_single_d<3>(P, st);
+    GR_LAUNCH_CHECK("dual_single_kernel");
+    return GR_OK;
 }
 
-int launch_primal(const TgvDims& d, float* u, float* ubar, float* p, float* pbar,
-                  const float* v, const float* w, const float* ktl, float tau, float norming,
-                  hipStream_t st)
+int launch_dual_batch(const ProbDev* probs, const DualItem* items, int nitems, int dim,
+                      int coupling, bool tgv, int maxc, hipStream_t st)
 {
-    dim3 blk(BX, BY), grd((d.nx + BX - 1) / BX, (d.ny + BY - 1) / BY, d.nz * d.nch);
-    const float hinv = 1.f / d.h;
-#define GR_PRIMAL(DM, TG)                                                                    \
-    hipLaunchKernelGGL((primal_kernel<DM, TG>), grd, blk, 0, st, u, ubar, p, pbar, v, w, ktl, \
-                       d.nx, d.ny, d.nz, d.nch, hinv, tau, norming)
-    if (d.dim == 2) {
-        if (d.tgv) GR_PRIMAL(2, true); else GR_PRIMAL(2, false);
-    } else {
-        if (d.tgv) GR_PRIMAL(3, true); else GR_PRIMAL(3, false);
+    if (nitems <= 0) return GR_OK;
+    if (maxc > 8) {
+        set_error("dual step: more than 8 channels per voxel");
+        return GR_ERR_UNSUPPORTED;
     }
-#undef GR_PRIMAL
-    GR_LAUNCH_CHECK("primal_kernel");
+    dim3 grd((unsigned)nitems), blk(DUAL_BX * DUAL_BY);
+#define GR_DB(DD, CC, TT, MM) \
+    hipLaunchKernelGGL((dual_batch_kernel<DD, CC, TT, MM>), grd, blk, 0, st, probs, items)
+#define GR_DB_M(DD, CC, TT) \
+    do { if (maxc <= 4) GR_DB(DD, CC, TT, 4); else GR_DB(DD, CC, TT, 8); } while (0)
+#define GR_DB_T(DD, CC) \
+    do { if (tgv) GR_DB_M(DD, CC, true); else GR_DB_M(DD, CC, false); } while (0)
+#define GR_DB_C(DD)                                                        \
+    do {                                                                   \
+        if (coupling == GR_COUPLING_UNCORRELATED) GR_DB_T(DD, GR_COUPLING_UNCORRELATED); \
+        else if (coupling == GR_COUPLING_FROBENIUS) GR_DB_T(DD, GR_COUPLING_FROBENIUS);  \
+        else GR_DB_T(DD, GR_COUPLING_NUCLEAR);                             \
+    } while (0)
+    if (dim == 2) GR_DB_C(2); else GR_DB_C(3);
+#undef GR_DB_C
+#undef GR_DB_T
+#undef GR_DB_M
+#undef GR_DB
+    GR_LAUNCH_CHECK("dual_batch_kernel");
     return GR_OK;
 }
 
 template <int DIM, int CPL, bool TGV>
-static void reg_launch(const TgvDims& d, const float* u, const float* p, double a0, double a1,
-                       double* partial, int nb, hipStream_t st)
+static void reg_launch(const ProbDev& P, double a0, double a1, double* partial, int nb,
+                       hipStream_t st)
 {
-    hipLaunchKernelGGL((reg_energy_kernel<DIM, CPL, TGV>), dim3(nb), dim3(EN_THREADS), 0, st, u,
-                       p, d.nx, d.ny, d.nz, d.nch, 1.0 / (double)d.h, a0, a1, partial);
+    hipLaunchKernelGGL((reg_energy_kernel<DIM, CPL, TGV>), dim3(nb), dim3(EN_THREADS), 0, st,
+                       P.u, P.p, P.nx, P.ny, P.nz, P.nch, (double)P.hinv, a0, a1, partial);
 }
 
-int launch_energy(const TgvDims& d, const float* u, const float* p, double a0, double a1,
-                  double* out_reg, hipStream_t st)
+int launch_energy(const ProbDev& P, double a0, double a1, double* out_reg, hipStream_t st)
 {
     const int nb = 1024;
     double* partial = nullptr;
@@ -583,18 +533,18 @@ int launch_energy(const TgvDims& d, const float* u, const float* p, double a0, d
         set_error("energy: device allocation failed");
         return GR_ERR_OOM;
     }
-#define GR_REG(DM, CP)                                                             \
-    do {                                                                           \
-        if (d.tgv) reg_launch<DM, CP, true>(d, u, p, a0, a1, partial, nb, st);     \
-        else reg_launch<DM, CP, false>(d, u, p, a0, a1, partial, nb, st);          \
+#define GR_REG(DM, CP)                                                   \
+    do {                                                                 \
+        if (P.tgv) reg_launch<DM, CP, true>(P, a0, a1, partial, nb, st); \
+        else reg_launch<DM, CP, false>(P, a0, a1, partial, nb, st);      \
     } while (0)
-    if (d.dim == 2) {
-        if (d.coupling == GR_COUPLING_UNCORRELATED) GR_REG(2, GR_COUPLING_UNCORRELATED);
-        else if (d.coupling == GR_COUPLING_FROBENIUS) GR_REG(2, GR_COUPLING_FROBENIUS);
+    if (P.dim == 2) {
+        if (P.coupling == GR_COUPLING_UNCORRELATED) GR_REG(2, GR_COUPLING_UNCORRELATED);
+        else if (P.coupling == GR_COUPLING_FROBENIUS) GR_REG(2, GR_COUPLING_FROBENIUS);
         else GR_REG(2, GR_COUPLING_NUCLEAR);
     } else {
-        if (d.coupling == GR_COUPLING_UNCORRELATED) GR_REG(3, GR_COUPLING_UNCORRELATED);
-        else if (d.coupling == GR_COUPLING_FROBENIUS) GR_REG(3, GR_COUPLING_FROBENIUS);
+        if (P.coupling == GR_COUPLING_UNCORRELATED) GR_REG(3, GR_COUPLING_UNCORRELATED);
+        else if (P.coupling == GR_COUPLING_FROBENIUS) GR_REG(3, GR_COUPLING_FROBENIUS);
         else GR_REG(3, GR_COUPLING_NUCLEAR);
     }
 #undef GR_REG

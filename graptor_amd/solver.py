@@ -148,7 +148,7 @@ class TgvProblem:
         self.v = z(d, *img)
         self.w = z(m, *img) if tgv else None
         self.lam = torch.zeros_like(self.f)
-        self.ktl = z(*img)
+        self.ktl = None  # K^T lam stays inside the fused backprojection kernel
         self.ku = torch.zeros_like(self.f)
         self.mfac = torch.as_tensor(self.k.mfac.astype(np.float32), device=self.device)
         self.params = _lib.TgvParams(
@@ -181,8 +181,9 @@ class TgvProblem:
 
     def profile(self, n: int):
         """Run ``n`` iterations with HIP events around each kernel; returns the
-        summed milliseconds of (dual, forward+prox, backprojection, primal)."""
-        out = (ctypes.c_double * 4)()
+        summed milliseconds of (dual step, forward projection + prox,
+        backprojection + primal step)."""
+        out = (ctypes.c_double * 3)()
         with torch.cuda.device(self.device):
             _lib.check(_lib.load().gr_tgv_profile(
                 self.geometry.handle, ctypes.byref(self.params), ctypes.byref(self.state),
@@ -206,6 +207,23 @@ class TgvProblem:
     def voxels(self) -> int:
         """Voxel-channels per iteration (the unit of voxel-iters/s)."""
         return self.C * self.Nz * self.nx * self.ny
+
+    def clone(self) -> "TgvProblem":
+        """An independent copy of the problem and its current state (same
+        geometry and constants); used for tuning on scratch state."""
+        other = object.__new__(TgvProblem)
+        other.__dict__.update({k: v for k, v in self.__dict__.items()
+                               if k not in ("params", "state")})
+        for name in ("f", "u", "ubar", "p", "pbar", "v", "w", "lam", "ktl", "ku", "mfac"):
+            t = getattr(self, name)
+            setattr(other, name, None if t is None else t.clone())
+        other.params = _lib.TgvParams.from_buffer_copy(self.params)
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        other.state = _lib.TgvState(
+            u=ptr(other.u), ubar=ptr(other.ubar), p=ptr(other.p), pbar=ptr(other.pbar),
+            v=ptr(other.v), w=ptr(other.w), lam=ptr(other.lam), ktl=ptr(other.ktl),
+            ku=ptr(other.ku), f=ptr(other.f), mfac=ptr(other.mfac))
+        return other
 
 
 def progress_line(start, width, i, maxiter, label):

@@ -106,19 +106,21 @@ typedef struct gr_tgv_state {
     const float *f, *mfac;
 } gr_tgv_state;
 
-/* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 (dual step,
- * projections, forward projection fused with the KL/L2 prox, backprojection,
- * primal step fused with the extrapolation) on `stream`.  With ku_on_last,
+/* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 on `stream`
+ * as three kernels: dual step with the coupled projections; forward
+ * projection fused with the KL/L2 prox; backprojection fused with the primal
+ * step and the extrapolation (K^T lam never reaches memory, so st->ktl may
+ * be NULL).  With ku_on_last,
  * the last iteration also stores R(ubar) into ku, the sinogram the reference
  * returns (2D:840-841). */
 int gr_tgv_iterate(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, int ku_on_last,
                    void *stream);
 
-/* gr_tgv_iterate with a HIP event pair around each of the four kernels of
+/* gr_tgv_iterate with a HIP event pair around each of the three kernels of
  * every iteration (measurement only; no counterpart in the reference).
- * ms_out[0..3] receive the summed kernel times in milliseconds of the dual
- * step, forward projection + prox, backprojection and primal step.
+ * ms_out[0..2] receive the summed kernel times in milliseconds of the dual
+ * step, the forward projection + prox and the backprojection + primal step.
  * Synchronises the stream. */
 int gr_tgv_profile(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, double *ms_out, void *stream);
@@ -132,6 +134,30 @@ int gr_tgv_profile(const gr_geometry *g, const gr_tgv_params *prm,
 int gr_tgv_energy(const gr_geometry *g, const gr_tgv_params *prm,
                   const gr_tgv_state *st, const double *mu, double alpha0,
                   double alpha1, double L, double *out, void *stream);
+
+/* Batches (no counterpart in the reference, which solves one problem per
+ * call): nprob independent problems with their own geometry, sizes,
+ * coupling, discrepancy and state, advanced together with one launch per
+ * kernel class and phase (at most a few launches per phase for the whole
+ * batch).  The plan keeps device copies of the descriptors; the state
+ * buffers stay caller-owned and must outlive the plan. */
+typedef struct gr_plan gr_plan;
+int gr_plan_create(int nprob, const gr_geometry *const *geoms,
+                   const gr_tgv_params *prms, const gr_tgv_state *states,
+                   gr_plan **out);
+int gr_plan_destroy(gr_plan *plan);
+/* n_iter iterations of every problem (ku_on_last as in gr_tgv_iterate). */
+int gr_plan_iterate(gr_plan *plan, int n_iter, int ku_on_last, void *stream);
+/* The same through a hipGraph of `block` iterations, captured once on a
+ * private stream and replayed on `stream` n_iter / block times; the
+ * remaining iterations run as ordinary launches. */
+int gr_plan_iterate_graph(gr_plan *plan, int n_iter, int block, void *stream);
+/* Per-phase kernel times (ms_out[3], as gr_tgv_profile) over n_iter
+ * iterations.  Synchronises the stream. */
+int gr_plan_profile(gr_plan *plan, int n_iter, double *ms_out, void *stream);
+/* Re-reads every geometry's tile shapes (gr_geometry_set_tuning) into the
+ * plan; used by the tile autotuner.  Drops a captured graph. */
+int gr_plan_retune(gr_plan *plan, const gr_geometry *const *geoms);
 
 #ifdef __cplusplus
 }
