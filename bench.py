@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ALPHA = (4.0, 1.0)
 MU = (0.2, 5e-4, 4e-3, 5e-4)
-KERNELS = ("dual_kernel", "radon_fwd_kernel", "radon_adj_kernel", "primal_kernel")
+KERNELS = ("dual_single_kernel", "fwd_single_kernel", "adj_single_kernel")
 
 
 def kernel_bytes(prob):
@@ -37,15 +37,13 @@ def kernel_bytes(prob):
     d = 2 if prob.dim == 2 else 3
     m = 3 if prob.dim == 2 else 6
     tgv = prob.regularisation == "TGV"
-    # dual: read ubar, pbar, v, w; write v, w
+    # dual step: read ubar, pbar, v, w; write v, w
     dual = V * 4 * (1 + (d if tgv else 0) + 2 * d + (2 * m if tgv else 0))
-    # forward + prox: read ubar; per bin read lam, f, write lam
+    # forward projection + prox: read ubar; per bin read lam, f, write lam
     fwd = V * 4 + B * 12
-    # backprojection: read lam, write K^T lam
-    adj = B * 4 + V * 4
-    # primal: read u, ktl, v, w, p; write u, ubar, p, pbar
-    primal = V * 4 * (2 + d + 2 + ((m + 3 * d) if tgv else 0))
-    return dict(zip(KERNELS, (dual, fwd, adj, primal)))
+    # backprojection + primal step: read lam; read u, v, w, p; write u, ubar, p, pbar
+    adj = B * 4 + V * 4 * (3 + d + ((m + 3 * d) if tgv else 0))
+    return dict(zip(KERNELS, (dual, fwd, adj)))
 
 
 def cpu_baseline(f_host, angles, L, iters):

@@ -1,0 +1,512 @@
+// BF16 MFMA kernels of the learned warm start (a small U-Net that maps
+// channel-wise backprojections to a TGV estimate used as the initial u).
+//
+// Tensors are NHWC bf16 with a pixel stride `ld` (channels allocated per
+// pixel) and a channel offset, so that a tensor can be a channel slice of a
+// wider buffer (U-Net skip concatenation without copies).  Weights are
+// W[co][tap][ci] (ci fastest), tap = 3*ky + kx for 3x3 (padding 1) or a
+// single tap for 1x1.  All products run on v_mfma_f32_32x32x16_bf16 with
+// FP32 accumulation.
+//
+// MFMA 32x32x16 operand layout (lane l = 0..63, h = l / 32):
+//   A[m][k]: lane holds m = l % 32, k = 8h .. 8h+7
+//   B[k][n]: lane holds n = l % 32, k = 8h .. 8h+7
+//   D[m][n]: lane holds n = l % 32, rows m = 8(r/4) + 4h + r%4, r = 0..15
+#include "common.h"
+
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace gr {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+
+struct alignas(16) U16x8 {
+    uint16_t v[8];
+};
+
+__device__ __forceinline__ bf16x8 as_bf16x8(const U16x8& u)
+{
+    bf16x8 r;
+    __builtin_memcpy(&r, &u, 16);
+    return r;
+}
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ---------------------------------------------------------------------------
+// Convolution forward as implicit GEMM: M = pixels (32 consecutive x of one
+// row per wave tile), N = output channels (32 per wave tile), K = taps x ci.
+// A work-group of 4 waves covers 64 pixels x 64 channels (2 x 2 wave tiles).
+// Requires cin % 16 == 0, cout % 32 == 0, width % 32 == 0, channel offsets
+// and strides multiples of 8.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void conv_fwd_kernel(
+    const uint16_t* __restrict__ x, int ldx, int cx0, const uint16_t* __restrict__ w,
+    const float* __restrict__ bias, void* __restrict__ y, int ldy, int cy0, int n, int H, int W,
+    int cin, int cout, int taps, int relu, int out_f32)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 5, l32 = lane & 31;
+    const int tiles_x = (W + 63) / 64;
+    const int co_blocks = (cout + 63) / 64;
+    // block -> (image, row, 64-pixel segment, 64-channel block)
+    int b = blockIdx.x;
+    const int cob = b % co_blocks;
+    b /= co_blocks;
+    const int seg = b % tiles_x;
+    b /= tiles_x;
+    const int row = b % H;
+    const int img = b / H;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int xbase = seg * 64 + wm * 32;   // first pixel x of this wave tile
+    const int co0 = cob * 64 + wn * 32;
+    if (co0 >= cout || xbase >= W) return;  // wave-uniform
+    const int px = xbase + l32;             // A row of this lane
+    const int co = co0 + l32;               // B column of this lane
+
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+    const int kpt = cin / 16;  // k-steps per tap
+    for (int t = 0; t < taps; ++t) {
+        const int dy = taps == 9 ? t / 3 - 1 : 0, dx = taps == 9 ? t % 3 - 1 : 0;
+        const int yy = row + dy, xx = px + dx;
+        const bool inb = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const uint16_t* xa =
+            x + ((size_t)(img * H + (inb ? yy : 0)) * W + (inb ? xx : 0)) * ldx + cx0 + 8 * h;
+        const uint16_t* wb = w + ((size_t)co * taps + t) * cin + 8 * h;
+        for (int kb = 0; kb < kpt; ++kb) {
+            U16x8 a, bb;
+            if (inb) a = *reinterpret_cast<const U16x8*>(xa + kb * 16);
+            else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) a.v[j] = 0;
+            }
+            bb = *reinterpret_cast<const U16x8*>(wb + kb * 16);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), as_bf16x8(bb), acc, 0, 0, 0);
+        }
+    }
+    // epilogue: lane holds column co, rows m = 8(r/4) + 4h + r%4
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = 8 * (r >> 2) + 4 * h + (r & 3);
+        const int xo = xbase + m;
+        float v = acc[r] + bv;
+        if (relu) v = v > 0.f ? v : 0.f;
+        const size_t pix = (size_t)(img * H + row) * W + xo;
+        if (out_f32)
+            reinterpret_cast<float*>(y)[pix * ldy + cy0 + co] = v;
+        else
+            reinterpret_cast<uint16_t*>(y)[pix * ldy + cy0 + co] = f2bf(v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[co][tap][ci] += sum over pixels of
+// dy[p][co] * x[p shifted by tap][ci].  M = co (32 per wave), N = ci (32 per
+// wave), K = pixels, staged through LDS transposed so both MFMA operands are
+// contiguous 16-byte reads.  A work-group covers 64 co x 64 ci of one tap
+// over a chunk of rows; partial sums are added to the FP32 accumulator with
+// atomics (split-K over the chunks).
+// ---------------------------------------------------------------------------
+constexpr int WG_PX = 32;  // pixels per k-step pair (two MFMAs of K = 16)
+
+__global__ __launch_bounds__(256) void conv_wgrad_kernel(
+    const uint16_t* __restrict__ x, int ldx, int cx0, const uint16_t* __restrict__ dy, int ldd,
+    int cd0, float* __restrict__ dw, int n, int H, int W, int cin, int cout, int taps,
+    int rows_per_chunk)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t sdy[64][WG_PX + 8];  // [co][px]
+    __shared__ __attribute__((aligned(16))) uint16_t sx[64][WG_PX + 8];   // [ci][px]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int h = lane >> 5, l32 = lane & 31;
+    const int co_blocks = (cout + 63) / 64, ci_blocks = (cin + 63) / 64;
+    int b = blockIdx.x;
+    const int cob = b % co_blocks; b /= co_blocks;
+    const int cib = b % ci_blocks; b /= ci_blocks;
+    const int t = b % taps; b /= taps;
+    const int chunk = b;  // chunk of rows over all images
+    const int total_rows = n * H;
+    const int r0 = chunk * rows_per_chunk;
+    const int r1 = min(total_rows, r0 + rows_per_chunk);
+    const int dyo = taps == 9 ? t / 3 - 1 : 0, dxo = taps == 9 ? t % 3 - 1 : 0;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+    const int tid = threadIdx.x;
+    for (int gr_ = r0; gr_ < r1; ++gr_) {
+        const int img = gr_ / H, row = gr_ - img * H;
+        const int yy = row + dyo;
+        for (int x0 = 0; x0 < W; x0 += WG_PX) {
+            __syncthreads();
+            // stage 32 pixels x 64 channels of dy and of shifted x (8 channels
+            // per thread: 32 px * 8 groups = 256 threads)
+            {
+                const int p = tid & 31, g = tid >> 5;  // g = 0..7 channel group
+                const int xo = x0 + p;
+                const int c = g * 8;
+                U16x8 vd, vx;
+                const int co = cob * 64 + c, ci = cib * 64 + c;
+                if (co < cout && xo < W)
+                    vd = *reinterpret_cast<const U16x8*>(
+                        dy + ((size_t)(img * H + row) * W + xo) * ldd + cd0 + co);
+                else
+                    for (int j = 0; j < 8; ++j) vd.v[j] = 0;
+                const int xs = xo + dxo;
+                if (ci < cin && xo < W && yy >= 0 && yy < H && xs >= 0 && xs < W)
+                    vx = *reinterpret_cast<const U16x8*>(
+                        x + ((size_t)(img * H + yy) * W + xs) * ldx + cx0 + ci);
+                else
+                    for (int j = 0; j < 8; ++j) vx.v[j] = 0;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    sdy[c + j][p] = vd.v[j];
+                    sx[c + j][p] = vx.v[j];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < WG_PX / 16; ++ks) {
+                U16x8 a = *reinterpret_cast<const U16x8*>(&sdy[wm * 32 + l32][ks * 16 + 8 * h]);
+                U16x8 bb = *reinterpret_cast<const U16x8*>(&sx[wn * 32 + l32][ks * 16 + 8 * h]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), as_bf16x8(bb), acc, 0,
+                                                              0, 0);
+            }
+        }
+    }
+    const int ci = cib * 64 + wn * 32 + l32;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int co = cob * 64 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        if (co < cout && ci < cin) atomicAdd(&dw[((size_t)co * taps + t) * cin + ci], acc[r]);
+    }
+}
+
+// db[co] += sum over pixels of dy[p][co] (FP32 accumulation, atomics per block)
+__global__ void bias_grad_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0, size_t npix,
+                                 int cout, float* __restrict__ db)
+{
+    const int co = threadIdx.x;  // blockDim.x == cout (<= 1024)
+    float acc = 0.f;
+    for (size_t p = blockIdx.x; p < npix; p += gridDim.x) acc += bf2f(dy[p * ldd + cd0 + co]);
+    atomicAdd(&db[co], acc);
+}
+
+// Wt[ci][8 - tap][co] = W[co][tap][ci]: weights of the data-gradient
+// convolution (transpose of channels, flip of taps).
+__global__ void weight_flip_kernel(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt,
+                                   int cin, int cout, int taps)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t total = (size_t)cin * cout * taps;
+    if (i >= total) return;
+    const int ci = (int)(i % cin);
+    const int t = (int)((i / cin) % taps);
+    const int co = (int)(i / ((size_t)cin * taps));
+    wt[((size_t)ci * taps + (taps - 1 - t)) * cout + co] = w[i];
+}
+
+// dy <- dy * (y > 0) (ReLU backward, y = forward output), in place
+__global__ void relu_bwd_kernel(uint16_t* __restrict__ dy, int ldd, int cd0,
+                                const uint16_t* __restrict__ y, int ldy, int cy0, size_t npix,
+                                int c)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix * c) return;
+    const size_t p = i / c;
+    const int ch = (int)(i % c);
+    if (!(bf2f(y[p * ldy + cy0 + ch]) > 0.f)) dy[p * ldd + cd0 + ch] = 0;
+}
+
+// 2x2 average pool (fwd) and its adjoint (bwd) on NHWC bf16
+__global__ void pool_fwd_kernel(const uint16_t* __restrict__ x, int ldx, int cx0,
+                                uint16_t* __restrict__ y, int ldy, int cy0, int n, int H, int W,
+                                int c)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * Ho * Wo * c) return;
+    const int ch = (int)(i % c);
+    size_t q = i / c;
+    const int xo = (int)(q % Wo); q /= Wo;
+    const int yo = (int)(q % Ho);
+    const int im = (int)(q / Ho);
+    float s = 0.f;
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx)
+            s += bf2f(x[((size_t)(im * H + 2 * yo + dy) * W + 2 * xo + dx) * ldx + cx0 + ch]);
+    y[((size_t)(im * Ho + yo) * Wo + xo) * ldy + cy0 + ch] = f2bf(0.25f * s);
+}
+
+// dx (fine) = 1/4 dy (coarse) broadcast; accumulate = add into dx
+__global__ void pool_bwd_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0,
+                                uint16_t* __restrict__ dx, int ldx, int cx0, int n, int H, int W,
+                                int c, int accumulate)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * H * W * c) return;
+    const int ch = (int)(i % c);
+    size_t q = i / c;
+    const int xf = (int)(q % W); q /= W;
+    const int yf = (int)(q % H);
+    const int im = (int)(q / H);
+    const int Ho = H / 2, Wo = W / 2;
+    float g = 0.25f * bf2f(dy[((size_t)(im * Ho + yf / 2) * Wo + xf / 2) * ldd + cd0 + ch]);
+    uint16_t* o = dx + ((size_t)(im * H + yf) * W + xf) * ldx + cx0 + ch;
+    if (accumulate) g += bf2f(*o);
+    *o = f2bf(g);
+}
+
+// nearest 2x upsample (fwd) and its adjoint (bwd: sum of the 2x2 block)
+__global__ void up_fwd_kernel(const uint16_t* __restrict__ x, int ldx, int cx0,
+                              uint16_t* __restrict__ y, int ldy, int cy0, int n, int H, int W,
+                              int c)  // H, W of the coarse input
+{
+    const int Ho = 2 * H, Wo = 2 * W;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * Ho * Wo * c) return;
+    const int ch = (int)(i % c);
+    size_t q = i / c;
+    const int xo = (int)(q % Wo); q /= Wo;
+    const int yo = (int)(q % Ho);
+    const int im = (int)(q / Ho);
+    y[((size_t)(im * Ho + yo) * Wo + xo) * ldy + cy0 + ch] =
+        x[((size_t)(im * H + yo / 2) * W + xo / 2) * ldx + cx0 + ch];
+}
+
+__global__ void up_bwd_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0,
+                              uint16_t* __restrict__ dx, int ldx, int cx0, int n, int H, int W,
+                              int c)  // H, W of the coarse gradient
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * H * W * c) return;
+    const int ch = (int)(i % c);
+    size_t q = i / c;
+    const int xc = (int)(q % W); q /= W;
+    const int yc = (int)(q % H);
+    const int im = (int)(q / H);
+    const int Wf = 2 * W, Hf = 2 * H;
+    float s = 0.f;
+    for (int a = 0; a < 2; ++a)
+        for (int bb = 0; bb < 2; ++bb)
+            s += bf2f(dy[((size_t)(im * Hf + 2 * yc + a) * Wf + 2 * xc + bb) * ldd + cd0 + ch]);
+    dx[((size_t)(im * H + yc) * W + xc) * ldx + cx0 + ch] = f2bf(s);
+}
+
+// y = bf16(x) for a dense float array (with output stride / offset per pixel)
+__global__ void pack_bf16_kernel(const float* __restrict__ x, int c_in, uint16_t* __restrict__ y,
+                                 int ldy, int cy0, size_t npix, float scale)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix * c_in) return;
+    const size_t p = i / c_in;
+    const int ch = (int)(i % c_in);
+    y[p * ldy + cy0 + ch] = f2bf(x[i] * scale);
+}
+
+// MSE loss gradient: g = (pred - target) * gscale as bf16 into an ld-strided
+// buffer (first c channels); loss partial sums into *loss (atomic)
+__global__ void mse_grad_kernel(const float* __restrict__ pred, int ldp,
+                                const float* __restrict__ target, int c, size_t npix,
+                                uint16_t* __restrict__ g, int ldg, float gscale,
+                                float* __restrict__ loss)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float d = 0.f;
+    if (i < npix * c) {
+        const size_t p = i / c;
+        const int ch = (int)(i % c);
+        d = pred[p * ldp + ch] - target[i];
+        g[p * ldg + ch] = f2bf(d * gscale);
+    }
+    float s = d * d;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
+}
+
+// Adam on FP32 master weights; also refreshes the bf16 copy
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v,
+                            uint16_t* __restrict__ pb, size_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2, float wd)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] + wd * p[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / c1, vh = vi / c2;
+    const float np = p[i] - lr * mh / (sqrtf(vh) + eps);
+    p[i] = np;
+    if (pb) pb[i] = f2bf(np);
+}
+
+}  // namespace
+
+}  // namespace gr
+
+using namespace gr;
+
+static hipStream_t cnn_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+static unsigned nblk(size_t n, int tb) { return (unsigned)((n + tb - 1) / tb); }
+
+extern "C" {
+
+int gr_cnn_conv_fwd(const void* x, int ldx, int cx0, const void* w, const float* bias, void* y,
+                    int ldy, int cy0, int n, int H, int W, int cin, int cout, int taps, int relu,
+                    int out_f32, void* stream)
+{
+    if (!x || !w || !y || n <= 0 || H <= 0 || W % 32 || cin % 16 || cout % 32 ||
+        (taps != 1 && taps != 9) || ldx % 8 || cx0 % 8 || cx0 + cin > ldx || cy0 + cout > ldy) {
+        set_error("cnn_conv_fwd: unsupported shape (cin %% 16, cout %% 32, W %% 32, 8-aligned ld)");
+        return GR_ERR_INVALID;
+    }
+    const int tiles_x = (W + 63) / 64;
+    const int co_blocks = (cout + 63) / 64;
+    const long long blocks = (long long)n * H * tiles_x * co_blocks;
+    hipLaunchKernelGGL(conv_fwd_kernel, dim3((unsigned)blocks), dim3(256), 0, cnn_stream(stream),
+                       (const uint16_t*)x, ldx, cx0, (const uint16_t*)w, bias, y, ldy, cy0, n, H,
+                       W, cin, cout, taps, relu, out_f32);
+    GR_LAUNCH_CHECK("conv_fwd_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_conv_wgrad(const void* x, int ldx, int cx0, const void* dy, int ldd, int cd0,
+                      float* dw, int n, int H, int W, int cin, int cout, int taps,
+                      int rows_per_chunk, void* stream)
+{
+    if (!x || !dy || !dw || n <= 0 || H <= 0 || W <= 0 || cin % 8 || cout % 8 || ldx % 8 ||
+        cx0 % 8 || ldd % 8 || cd0 % 8 || (taps != 1 && taps != 9) || rows_per_chunk <= 0) {
+        set_error("cnn_conv_wgrad: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    const int chunks = (n * H + rows_per_chunk - 1) / rows_per_chunk;
+    const long long blocks =
+        (long long)((cout + 63) / 64) * ((cin + 63) / 64) * taps * chunks;
+    hipLaunchKernelGGL(conv_wgrad_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (const uint16_t*)dy, ldd,
+                       cd0, dw, n, H, W, cin, cout, taps, rows_per_chunk);
+    GR_LAUNCH_CHECK("conv_wgrad_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_bias_grad(const void* dy, int ldd, int cd0, size_t npix, int cout, float* db,
+                     void* stream)
+{
+    if (!dy || !db || cout <= 0 || cout > 1024) {
+        set_error("cnn_bias_grad: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    const unsigned blocks = (unsigned)std::min<size_t>(1024, npix);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks), dim3(cout), 0, cnn_stream(stream),
+                       (const uint16_t*)dy, ldd, cd0, npix, cout, db);
+    GR_LAUNCH_CHECK("bias_grad_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_weight_flip(const void* w, void* wt, int cin, int cout, int taps, void* stream)
+{
+    const size_t total = (size_t)cin * cout * taps;
+    hipLaunchKernelGGL(weight_flip_kernel, dim3(nblk(total, 256)), dim3(256), 0,
+                       cnn_stream(stream), (const uint16_t*)w, (uint16_t*)wt, cin, cout, taps);
+    GR_LAUNCH_CHECK("weight_flip_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_relu_bwd(void* dy, int ldd, int cd0, const void* y, int ldy, int cy0, size_t npix,
+                    int c, void* stream)
+{
+    hipLaunchKernelGGL(relu_bwd_kernel, dim3(nblk(npix * c, 256)), dim3(256), 0,
+                       cnn_stream(stream), (uint16_t*)dy, ldd, cd0, (const uint16_t*)y, ldy, cy0,
+                       npix, c);
+    GR_LAUNCH_CHECK("relu_bwd_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_pool(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, int n, int H, int W,
+                int c, int backward, int accumulate, void* stream)
+{
+    if (H % 2 || W % 2) {
+        set_error("cnn_pool: even sizes required");
+        return GR_ERR_INVALID;
+    }
+    if (!backward)
+        hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk((size_t)n * H / 2 * W / 2 * c, 256)),
+                           dim3(256), 0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0,
+                           (uint16_t*)y, ldy, cy0, n, H, W, c);
+    else  // x = coarse gradient, y = fine gradient (H, W fine)
+        hipLaunchKernelGGL(pool_bwd_kernel, dim3(nblk((size_t)n * H * W * c, 256)), dim3(256), 0,
+                           cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
+                           cy0, n, H, W, c, accumulate);
+    GR_LAUNCH_CHECK("pool");
+    return GR_OK;
+}
+
+int gr_cnn_upsample(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, int n, int H,
+                    int W, int c, int backward, void* stream)
+{
+    if (!backward)
+        hipLaunchKernelGGL(up_fwd_kernel, dim3(nblk((size_t)n * 4 * H * W * c, 256)), dim3(256),
+                           0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
+                           cy0, n, H, W, c);
+    else  // x = fine gradient, y = coarse gradient (H, W coarse)
+        hipLaunchKernelGGL(up_bwd_kernel, dim3(nblk((size_t)n * H * W * c, 256)), dim3(256), 0,
+                           cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
+                           cy0, n, H, W, c);
+    GR_LAUNCH_CHECK("upsample");
+    return GR_OK;
+}
+
+int gr_cnn_pack_bf16(const float* x, int c_in, void* y, int ldy, int cy0, size_t npix,
+                     float scale, void* stream)
+{
+    hipLaunchKernelGGL(pack_bf16_kernel, dim3(nblk(npix * c_in, 256)), dim3(256), 0,
+                       cnn_stream(stream), x, c_in, (uint16_t*)y, ldy, cy0, npix, scale);
+    GR_LAUNCH_CHECK("pack_bf16_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_mse_grad(const float* pred, int ldp, const float* target, int c, size_t npix, void* g,
+                    int ldg, float gscale, float* loss, void* stream)
+{
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(nblk(npix * c, 256)), dim3(256), 0,
+                       cnn_stream(stream), pred, ldp, target, c, npix, (uint16_t*)g, ldg, gscale,
+                       loss);
+    GR_LAUNCH_CHECK("mse_grad_kernel");
+    return GR_OK;
+}
+
+int gr_cnn_adam(float* p, const float* g, float* m, float* v, void* pb, size_t n, float lr,
+                float b1, float b2, float eps, int step, float wd, void* stream)
+{
+    const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(nblk(n, 256)), dim3(256), 0, cnn_stream(stream), p, g, m,
+                       v, (uint16_t*)pb, n, lr, b1, b2, eps, c1, c2, wd);
+    GR_LAUNCH_CHECK("adam_kernel");
+    return GR_OK;
+}
+
+}  // extern "C"
