@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ALPHA = (4.0, 1.0)
 MU = (0.2, 5e-4, 4e-3, 5e-4)
-KERNELS = ("dual_single_kernel", "fwd_single_kernel", "adj_single_kernel")
+KERNELS = ("dual_single_kernel", "fwd_single_kernel", "adj_single_kernel",
+           "primal_single_kernel")
 
 
 def kernel_bytes(prob):
@@ -41,9 +42,11 @@ def kernel_bytes(prob):
     dual = V * 4 * (1 + (d if tgv else 0) + 2 * d + (2 * m if tgv else 0))
     # forward projection + prox: read ubar; per bin read lam, f, write lam
     fwd = V * 4 + B * 12
-    # backprojection + primal step: read lam; read u, v, w, p; write u, ubar, p, pbar
-    adj = B * 4 + V * 4 * (3 + d + ((m + 3 * d) if tgv else 0))
-    return dict(zip(KERNELS, (dual, fwd, adj)))
+    # backprojection: read lam, write K^T lam
+    adj = B * 4 + V * 4
+    # primal step: read u, K^T lam, v, w, p; write u, ubar, p, pbar
+    primal = V * 4 * (2 + d + 2 + ((m + 3 * d) if tgv else 0))
+    return dict(zip(KERNELS, (dual, fwd, adj, primal)))
 
 
 def cpu_baseline(f_host, angles, L, iters):
@@ -90,6 +93,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (known bytes) so that "
+                         "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE can be scaled")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -136,6 +142,16 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     achieved = kern[dom]["gbs"]
     print("# kernels: " + json.dumps(kern), file=sys.stderr, flush=True)
+
+    if args.copy_calibration:
+        src = torch.ones(256 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# copy calibration: 3 copies of 1 GiB (read 1 GiB, write 1 GiB each)",
+              file=sys.stderr, flush=True)
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -27,6 +27,7 @@ NORMEST_MODE = {"reference": 0, "power": 1}
 EXPORTS = (
     "gr_last_error", "gr_abi_version", "gr_radon_struct", "gr_geometry_create",
     "gr_geometry_destroy", "gr_geometry_set_tuning", "gr_geometry_get_tuning",
+    "gr_geometry_set_fusion",
     "gr_radon", "gr_radon_ad", "gr_normest", "gr_make_feasible", "gr_tgv_iterate",
     "gr_tgv_energy", "gr_tgv_profile", "gr_plan_create", "gr_plan_destroy",
     "gr_plan_iterate", "gr_plan_iterate_graph", "gr_plan_profile", "gr_plan_retune",
@@ -85,6 +86,7 @@ def load(path: str | None = None):
             "gr_geometry_destroy": ([V], I),
             "gr_geometry_set_tuning": ([V, I, I], I),
             "gr_geometry_get_tuning": ([V, PI, PI], I),
+            "gr_geometry_set_fusion": ([V, I], I),
             "gr_radon": ([V, V, V, I, V], I),
             "gr_radon_ad": ([V, V, V, I, V], I),
             "gr_normest": ([V, V, I, I, D, PD, V], I),

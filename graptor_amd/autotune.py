@@ -60,7 +60,8 @@ def autotune(problems, iters=3, plan=None, cache=None):
         for g in geoms.values():
             g.set_tuning(fwd_band=best_band, adj_tile_y=tile)
         splan.retune()
-        adj_ms[tile] = splan.profile(iters)[2] / iters
+        prof = splan.profile(iters)
+        adj_ms[tile] = (prof[2] + prof[3]) / iters
     best_tile = min(adj_ms, key=adj_ms.get)
     del splan, scratch
     _apply(problems, best_band, best_tile, plan)

@@ -57,8 +57,8 @@ class BatchPlan:
             p.iterations += n
 
     def profile(self, n: int):
-        """Summed ms of (dual, forward+prox, backprojection+primal) phases."""
-        out = (ctypes.c_double * 3)()
+        """Summed ms of (dual, forward+prox, backprojection, primal) phases."""
+        out = (ctypes.c_double * 4)()
         with torch.cuda.device(self.device):
             _lib.check(_lib.load().gr_plan_profile(self._h, int(n), out, self._stream()),
                        "plan_profile")

@@ -86,10 +86,12 @@ bool valid_params(const gr_geometry* g, const gr_tgv_params* p)
     return true;
 }
 
-bool valid_state(const gr_tgv_params* p, const gr_tgv_state* s, bool need_ku)
+bool valid_state(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_state* s,
+                 bool need_ku)
 {
     const bool tgv = p->regularisation == GR_REG_TGV;
     if (!s || !s->u || !s->ubar || !s->v || !s->lam || !s->f || !s->mfac) return false;
+    if (!g->g.fuse_primal && !s->ktl) return false;
     if (tgv && (!s->p || !s->pbar || !s->w)) return false;
     if (need_ku && !s->ku) return false;
     return true;
@@ -115,6 +117,7 @@ ProbDev make_prob(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
     P.adj_tile_y = g.adj_tile_y;
     P.adj_segw = adj_segw_of(P.adj_tile_y);
     P.adj_achunk = adj_achunk_of(g.na, P.adj_segw, K);
+    P.fuse_primal = g.fuse_primal;
     P.ofs = reinterpret_cast<const float4*>(g.ofs_dev);
     P.fwd = g.fwd_dev;
     P.u = s->u; P.ubar = s->ubar; P.p = s->p; P.pbar = s->pbar; P.v = s->v; P.w = s->w;
@@ -127,10 +130,10 @@ int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_o
 {
     const int nsl = P.nz * P.nch;
     const int pmode = P.disc == GR_DISC_KL ? FWD_KL : FWD_L2;
-    hipEvent_t ev[4];
+    hipEvent_t ev[5];
     if (ms_out) {
-        for (int k = 0; k < 4; ++k) GR_HIP(hipEventCreate(&ev[k]));
-        for (int k = 0; k < 3; ++k) ms_out[k] = 0.0;
+        for (int k = 0; k < 5; ++k) GR_HIP(hipEventCreate(&ev[k]));
+        for (int k = 0; k < 4; ++k) ms_out[k] = 0.0;
     }
     int rc = GR_OK;
     for (int it = 0; it < n_iter && rc == GR_OK; ++it) {
@@ -140,12 +143,14 @@ int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_o
         const int mode = pmode | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
         if (!rc) rc = launch_fwd(P, gg->g, nsl, mode, st);
         if (ms_out) (void)hipEventRecord(ev[2], st);
-        if (!rc) rc = launch_adj(P, nsl, true, st);
+        if (!rc) rc = launch_adj(P, nsl, P.fuse_primal != 0, st);
+        if (ms_out) (void)hipEventRecord(ev[3], st);
+        if (!rc && !P.fuse_primal) rc = launch_primal(P, st);
         if (ms_out) {
-            (void)hipEventRecord(ev[3], st);
-            hipError_t e = hipEventSynchronize(ev[3]);
+            (void)hipEventRecord(ev[4], st);
+            hipError_t e = hipEventSynchronize(ev[4]);
             if (e != hipSuccess && !rc) rc = hip_status(e, "tgv_profile");
-            for (int k = 0; k < 3 && !rc; ++k) {
+            for (int k = 0; k < 4 && !rc; ++k) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
                 ms_out[k] += ms;
@@ -153,7 +158,7 @@ int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_o
         }
     }
     if (ms_out)
-        for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
+        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
     return rc;
 }
 
@@ -167,11 +172,13 @@ struct gr_plan {
     std::vector<ProbDev> probs;
     ProbDev* probs_dev = nullptr;
     struct FwdLaunch { int K, mi; size_t lds; std::vector<FwdItem> items; FwdItem* dev; };
-    struct AdjLaunch { int K, dim, tgv; size_t lds; std::vector<AdjItem> items; AdjItem* dev; };
+    struct AdjLaunch { int K, dim, tgv, fuse; size_t lds; std::vector<AdjItem> items; AdjItem* dev; };
+    struct PrimalLaunch { int dim, tgv; std::vector<DualItem> items; DualItem* dev; };
     struct DualLaunch { int dim, coupling, tgv, maxc; std::vector<DualItem> items; DualItem* dev; };
     std::vector<FwdLaunch> fwd;
     std::vector<AdjLaunch> adj;
     std::vector<DualLaunch> dual;
+    std::vector<PrimalLaunch> primal;
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0;
@@ -186,6 +193,7 @@ void plan_free_device(gr_plan* pl)
     for (auto& l : pl->fwd) if (l.dev) (void)hipFree(l.dev);
     for (auto& l : pl->adj) if (l.dev) (void)hipFree(l.dev);
     for (auto& l : pl->dual) if (l.dev) (void)hipFree(l.dev);
+    for (auto& l : pl->primal) if (l.dev) (void)hipFree(l.dev);
     if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
     pl->graph = nullptr;
     pl->graph_block = 0;
@@ -207,22 +215,30 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
     pl->fwd.clear();
     pl->adj.clear();
     pl->dual.clear();
+    pl->primal.clear();
     const int n = pl->nprob;
-    // forward chunk table: fewest items per thread giving >= 1024 work-groups
+    // forward chunk table: the items per thread whose batch grid is closest
+    // to three work-groups per CU
     int t = 0;
-    for (; t < 3; ++t) {
+    double best_d = 1e30;
+    for (int tt = 0; tt < 4; ++tt) {
         long long total = 0;
         for (int i = 0; i < n; ++i) {
             const Geometry& g = geoms[i]->g;
             const int S = pl->probs[i].nz * pl->probs[i].nch;
             const int K = slice_group(S);
-            total += (long long)((S + K - 1) / K) * (g.tab_off[t + 1] - g.tab_off[t]);
+            total += (long long)((S + K - 1) / K) * (g.tab_off[tt + 1] - g.tab_off[tt]);
         }
-        if (total >= 1024) break;
+        const double d = std::fabs(std::log(std::max((double)total, 1.0) / 768.0));
+        if (d < best_d - 1e-9) {
+            best_d = d;
+            t = tt;
+        }
     }
     const int mi = 8 >> t;
     std::map<std::tuple<int, int>, size_t> fwd_idx;
-    std::map<std::tuple<int, int, int>, size_t> adj_idx;
+    std::map<std::tuple<int, int, int, int>, size_t> adj_idx;
+    std::map<std::tuple<int, int>, size_t> primal_idx;
     std::map<std::tuple<int, int, int, int>, size_t> dual_idx;
     for (int i = 0; i < n; ++i) {
         const ProbDev& P = pl->probs[i];
@@ -243,16 +259,29 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
                 fl.items.push_back(FwdItem{i, ch.cls, ch.a0, ch.na, ch.b0, ch.nb, s0, 0});
             }
         // backprojection + primal items: tiles x slice groups
-        auto ak = std::make_tuple(K, P.dim, P.tgv);
+        auto ak = std::make_tuple(K, P.dim, P.tgv, P.fuse_primal);
         if (!adj_idx.count(ak)) {
             adj_idx[ak] = pl->adj.size();
-            pl->adj.push_back({K, P.dim, P.tgv, 0, {}, nullptr});
+            pl->adj.push_back({K, P.dim, P.tgv, P.fuse_primal, 0, {}, nullptr});
         }
         auto& al = pl->adj[adj_idx[ak]];
         al.lds = std::max(al.lds, adj_lds_bytes(P, K));
         for (int s0 = 0; s0 < S; s0 += K)
             for (int y0 = 0; y0 < P.ny; y0 += P.adj_tile_y)
                 for (int x0 = 0; x0 < P.nx; x0 += ADJ_TX) al.items.push_back(AdjItem{i, x0, y0, s0});
+        // unfused primal items: 64 x 4 tiles of every slice of every channel
+        if (!P.fuse_primal) {
+            auto pk = std::make_tuple(P.dim, P.tgv);
+            if (!primal_idx.count(pk)) {
+                primal_idx[pk] = pl->primal.size();
+                pl->primal.push_back({P.dim, P.tgv, {}, nullptr});
+            }
+            auto& plp = pl->primal[primal_idx[pk]];
+            for (int sl = 0; sl < S; ++sl)
+                for (int y0 = 0; y0 < P.ny; y0 += DUAL_BY)
+                    for (int x0 = 0; x0 < P.nx; x0 += DUAL_BX)
+                        plp.items.push_back(DualItem{i, x0, y0, sl});
+        }
         // dual items: 64 x 4 tiles of every plane
         const int maxc = P.nch <= 4 ? 4 : 8;
         auto dk = std::make_tuple(P.dim, P.coupling, P.tgv, maxc);
@@ -274,6 +303,7 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
     for (auto& l : pl->fwd) if (!rc) rc = upload(l.items, &l.dev);
     for (auto& l : pl->adj) if (!rc) rc = upload(l.items, &l.dev);
     for (auto& l : pl->dual) if (!rc) rc = upload(l.items, &l.dev);
+    for (auto& l : pl->primal) if (!rc) rc = upload(l.items, &l.dev);
     return rc;
 }
 
@@ -288,10 +318,14 @@ int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
         for (auto& l : pl->fwd)
             if (!rc) rc = launch_fwd_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.mi,
                                            l.lds, fwd_mode, st);
-    } else {
+    } else if (phase == 2) {
         for (auto& l : pl->adj)
             if (!rc) rc = launch_adj_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.dim,
-                                           l.tgv != 0, l.lds, st);
+                                           l.tgv != 0, l.fuse != 0, l.lds, st);
+    } else {
+        for (auto& l : pl->primal)
+            if (!rc) rc = launch_primal_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim,
+                                              l.tgv != 0, st);
     }
     return rc;
 }
@@ -303,6 +337,7 @@ int plan_iter(gr_plan* pl, int n_iter, int ku_on_last, hipStream_t st)
         int rc = plan_phase(pl, 0, mode, st);
         if (!rc) rc = plan_phase(pl, 1, mode, st);
         if (!rc) rc = plan_phase(pl, 2, mode, st);
+        if (!rc) rc = plan_phase(pl, 3, mode, st);
         if (rc) return rc;
     }
     return GR_OK;
@@ -369,7 +404,8 @@ int gr_geometry_create(const float* ofs, int na, int nx, int ny, int nd, gr_geom
     Geometry& g = gg->g;
     g.nx = nx; g.ny = ny; g.nd = nd; g.na = na;
     g.fwd_band = 0;
-    g.adj_tile_y = 16;
+    g.adj_tile_y = 32;
+    g.fuse_primal = 0;
     g.ofs_dev = nullptr; g.fwd_dev = nullptr; g.chunks_dev = nullptr;
     std::vector<FwdAngle> fa(na);
     for (int j = 0; j < na; ++j) {
@@ -445,6 +481,16 @@ int gr_geometry_set_tuning(gr_geometry* gg, int fwd_band, int adj_tile_y)
     return GR_OK;
 }
 
+int gr_geometry_set_fusion(gr_geometry* gg, int fuse_primal)
+{
+    if (!gg || fuse_primal < 0 || fuse_primal > 1) {
+        set_error("set_fusion: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    gg->g.fuse_primal = fuse_primal;
+    return GR_OK;
+}
+
 int gr_geometry_get_tuning(const gr_geometry* gg, int* fwd_band, int* adj_tile_y)
 {
     if (!gg) {
@@ -503,7 +549,7 @@ int gr_make_feasible(const gr_geometry* gg, float* f, int nslices, void* stream)
 int gr_tgv_iterate(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
                    int n_iter, int ku_on_last, void* stream)
 {
-    if (!valid_params(gg, p) || n_iter < 0 || !valid_state(p, s, ku_on_last != 0)) {
+    if (!valid_params(gg, p) || n_iter < 0 || !valid_state(gg, p, s, ku_on_last != 0)) {
         set_error("tgv_iterate: invalid parameters or missing state buffer");
         return GR_ERR_INVALID;
     }
@@ -515,7 +561,7 @@ int gr_tgv_iterate(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_s
 int gr_tgv_profile(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
                    int n_iter, double* ms_out, void* stream)
 {
-    if (!valid_params(gg, p) || n_iter <= 0 || !ms_out || !valid_state(p, s, false)) {
+    if (!valid_params(gg, p) || n_iter <= 0 || !ms_out || !valid_state(gg, p, s, false)) {
         set_error("tgv_profile: invalid parameters or missing state buffer");
         return GR_ERR_INVALID;
     }
@@ -565,7 +611,7 @@ int gr_plan_create(int nprob, const gr_geometry* const* geoms, const gr_tgv_para
     }
     *out = nullptr;
     for (int i = 0; i < nprob; ++i)
-        if (!valid_params(geoms[i], &prms[i]) || !valid_state(&prms[i], &states[i], false)) {
+        if (!valid_params(geoms[i], &prms[i]) || !valid_state(geoms[i], &prms[i], &states[i], false)) {
             set_error("plan_create: problem %d has invalid parameters or state", i);
             return GR_ERR_INVALID;
         }
@@ -653,25 +699,25 @@ int gr_plan_profile(gr_plan* pl, int n_iter, double* ms_out, void* stream)
         return GR_ERR_INVALID;
     }
     hipStream_t st = stream_of(stream);
-    hipEvent_t ev[4];
-    for (int k = 0; k < 4; ++k) GR_HIP(hipEventCreate(&ev[k]));
-    for (int k = 0; k < 3; ++k) ms_out[k] = 0.0;
+    hipEvent_t ev[5];
+    for (int k = 0; k < 5; ++k) GR_HIP(hipEventCreate(&ev[k]));
+    for (int k = 0; k < 4; ++k) ms_out[k] = 0.0;
     int rc = GR_OK;
     for (int it = 0; it < n_iter && !rc; ++it) {
-        for (int ph = 0; ph < 3 && !rc; ++ph) {
+        for (int ph = 0; ph < 4 && !rc; ++ph) {
             (void)hipEventRecord(ev[ph], st);
             rc = plan_phase(pl, ph, FWD_PROX, st);
         }
-        (void)hipEventRecord(ev[3], st);
-        hipError_t e = hipEventSynchronize(ev[3]);
+        (void)hipEventRecord(ev[4], st);
+        hipError_t e = hipEventSynchronize(ev[4]);
         if (e != hipSuccess && !rc) rc = hip_status(e, "plan_profile");
-        for (int k = 0; k < 3 && !rc; ++k) {
+        for (int k = 0; k < 4 && !rc; ++k) {
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
             ms_out[k] += ms;
         }
     }
-    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
     return rc;
 }
 
@@ -689,6 +735,7 @@ int gr_plan_retune(gr_plan* pl, const gr_geometry* const* geoms)
         P.adj_tile_y = g.adj_tile_y;
         P.adj_segw = adj_segw_of(P.adj_tile_y);
         P.adj_achunk = adj_achunk_of(g.na, P.adj_segw, K);
+        P.fuse_primal = g.fuse_primal;
     }
     return plan_build(pl, geoms);
 }

@@ -56,7 +56,7 @@ struct ProbDev {
     int dim, nx, ny, nz, nch, nd, na;
     int coupling, tgv, disc;
     float hinv, sigma, tau, sigma_p, norming, ainv0, ainv1;
-    int fwd_band, fwd_pitch, adj_tile_y, adj_segw, adj_achunk;
+    int fwd_band, fwd_pitch, adj_tile_y, adj_segw, adj_achunk, fuse_primal;
     const float4* ofs;     // na x (cos, sin, off, 1/cos)
     const FwdAngle* fwd;   // na, sorted by class
     // state; for standalone projector calls: ubar = image in, ku = sinogram
@@ -67,6 +67,8 @@ struct ProbDev {
 
 struct FwdItem { int prob, cls, a0, na, b0, nb, s0, pad; };
 struct AdjItem { int prob, x0, y0, s0; };
+// Stencil items: a 64 x 4 tile of plane z (dual step) or of slice
+// z + nz * channel (primal step).
 struct DualItem { int prob, x0, y0, z; };
 
 // Device-side geometry (owned by gr_geometry).
@@ -80,6 +82,7 @@ struct Geometry {
     std::vector<FwdChunk> chunks_host;
     int fwd_band;            // lines per LDS band (0: automatic)
     int adj_tile_y;          // rows per backprojection tile
+    int fuse_primal;         // 1: primal step in the backprojection epilogue
 };
 
 // Modes of the forward projector epilogue.
@@ -95,6 +98,7 @@ constexpr int DUAL_BX = 64, DUAL_BY = 4;
 
 // Kernel-side sizes derived from a problem and its tuning.
 int fwd_pitch_of(int nx, int ny);
+int fwd_table_choice(const Geometry& g, long long groups);
 int fwd_band_auto(int pitch, int K);
 int adj_segw_of(int tile_y);
 int adj_achunk_of(int na, int segw, int K);
@@ -108,12 +112,15 @@ int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipSt
 // the primal step of the iteration (primal == true)
 int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st);
 int launch_dual(const ProbDev& P, hipStream_t st);
+int launch_primal(const ProbDev& P, hipStream_t st);
 
 // ---- launchers (batches: descriptor table + item table) -------------------
 int launch_fwd_batch(const ProbDev* probs_dev, const FwdItem* items, int nitems, int K,
                      int mi, size_t lds, int mode, hipStream_t st);
 int launch_adj_batch(const ProbDev* probs_dev, const AdjItem* items, int nitems, int K,
-                     int dim, bool tgv, size_t lds, hipStream_t st);
+                     int dim, bool tgv, bool primal, size_t lds, hipStream_t st);
+int launch_primal_batch(const ProbDev* probs_dev, const DualItem* items, int nitems, int dim,
+                        bool tgv, hipStream_t st);
 int launch_dual_batch(const ProbDev* probs_dev, const DualItem* items, int nitems, int dim,
                       int coupling, bool tgv, int maxc, hipStream_t st);
 

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace gr {
 
@@ -227,13 +228,13 @@ __global__ __launch_bounds__(FWD_THREADS) void fwd_batch_kernel(const ProbDev* _
 // i indexes u[c][z][y][x] (= slice s = z + nz c), z = s % nz.
 // --------------------------------------------------------------------------
 template <int DIM, bool TGV>
-__device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x, int y, int z,
-                                              float ktl)
+__device__ __forceinline__ void primal_update_r(
+    float* __restrict__ u, float* __restrict__ ubar, float* __restrict__ p,
+    float* __restrict__ pbar, const float* __restrict__ v, const float* __restrict__ w, int nx,
+    int ny, int nz, size_t cs, float hinv, float tau, float norming, size_t i, int x, int y, int z,
+    float ktl)
 {
-    const int nx = P.nx, ny = P.ny, nz = P.nz;
     const size_t plane = (size_t)nx * ny;
-    const size_t cs = plane * nz * P.nch;  // component stride
-    const float* __restrict__ v = P.v;
     float vc[3];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
@@ -245,21 +246,17 @@ __device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x,
     if constexpr (DIM == 3) {
         float d2 = (z == nz - 1) ? 0.f : vc[2];
         if (z > 0) d2 -= v[i + 2 * cs - plane];
-        dv += d2 * P.hinv;
+        dv += d2 * hinv;
     }
-    const float uo = P.u[i];
-    float un = uo + P.tau * (dv - P.norming * ktl);
-    un = un < 0.f ? 0.f : un;
-    P.u[i] = un;
-    P.ubar[i] = 2.f * un - uo;
-
+    float q[3] = {0.f, 0.f, 0.f};
+    float po[3] = {0.f, 0.f, 0.f};
     if constexpr (TGV) {
-        const float* __restrict__ w = P.w;
         constexpr int M = DIM == 2 ? 3 : 6;
         float wc[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) wc[m] = w[i + m * cs];
-        float q[3];
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) po[k] = p[i + k * cs];
         if constexpr (DIM == 2) {
             // x-part uses (w_xx, w_xy), y-part (w_xy, w_yy)
             float x0 = (x == 0) ? 0.f : -wc[0], x1 = (x == 0) ? 0.f : -wc[2];
@@ -280,17 +277,64 @@ __device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x,
                 if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
                 float zs = (z == 0) ? 0.f : -wc[zi[k]];
                 if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
-                q[k] = xs + ys + zs * P.hinv;
+                q[k] = xs + ys + zs * hinv;
             }
         }
+    }
+    const float uo = u[i];
+    float un = uo + tau * (dv - norming * ktl);
+    un = un < 0.f ? 0.f : un;
+    u[i] = un;
+    ubar[i] = 2.f * un - uo;
+    if constexpr (TGV) {
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            const float po = P.p[i + k * cs];
-            const float pn = po + P.tau * (vc[k] + q[k]);
-            P.p[i + k * cs] = pn;
-            P.pbar[i + k * cs] = 2.f * pn - po;
+            const float pn = po[k] + tau * (vc[k] + q[k]);
+            p[i + k * cs] = pn;
+            pbar[i + k * cs] = 2.f * pn - po[k];
         }
     }
+}
+
+template <int DIM, bool TGV>
+__device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x, int y, int z,
+                                              float ktl)
+{
+    const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;  // component stride
+    primal_update_r<DIM, TGV>(P.u, P.ubar, P.p, P.pbar, P.v, P.w, P.nx, P.ny, P.nz, cs, P.hinv,
+                              P.tau, P.norming, i, x, y, z, ktl);
+}
+
+// Primal step of one voxel-channel per thread with K^T lam read from memory
+// (the unfused form: high occupancy for the memory-bound update).
+template <int DIM, bool TGV>
+__device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
+{
+    const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
+    const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
+    const int s = I.z;  // slice z + nz * c
+    if (x >= P.nx || y >= P.ny) return;
+    const size_t i = (size_t)s * P.nx * P.ny + (size_t)y * P.nx + x;
+    primal_update<DIM, TGV>(P, i, x, y, s % P.nz, P.ktl[i]);
+}
+
+template <int DIM, bool TGV>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_single_kernel(ProbDev P, int tiles_x,
+                                                                         int tiles_y)
+{
+    const int b = blockIdx.x;
+    const int tx = b % tiles_x, rest = b / tiles_x;
+    const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
+    primal_body<DIM, TGV>(P, I);
+}
+
+template <int DIM, bool TGV>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_batch_kernel(
+    const ProbDev* __restrict__ probs, const DualItem* __restrict__ items)
+{
+    const DualItem I = items[blockIdx.x];
+    const ProbDev P = probs[I.prob];
+    primal_body<DIM, TGV>(P, I);
 }
 
 // --------------------------------------------------------------------------
@@ -405,14 +449,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_single_kernel(ProbDev P, int 
     adj_body<K, PRIMAL, DIM, TGV>(P, I, sh);
 }
 
-template <int K, int DIM, bool TGV>
+template <int K, bool PRIMAL, int DIM, bool TGV>
 __global__ __launch_bounds__(ADJ_THREADS) void adj_batch_kernel(const ProbDev* __restrict__ probs,
                                                                 const AdjItem* __restrict__ items)
 {
     extern __shared__ float sh[];
     const AdjItem I = items[xcd_work_id(blockIdx.x, gridDim.x)];
     const ProbDev P = probs[I.prob];
-    adj_body<K, true, DIM, TGV>(P, I, sh);
+    adj_body<K, PRIMAL, DIM, TGV>(P, I, sh);
 }
 
 }  // namespace
@@ -420,6 +464,22 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_batch_kernel(const ProbDev* _
 // --------------------------------------------------------------------------
 // sizes
 // --------------------------------------------------------------------------
+int fwd_table_choice(const Geometry& g, long long groups)
+{
+    constexpr double target = 3.0 * 256;  // work-groups: ~3 per CU on 256 CUs
+    int best = 0;
+    double best_d = 1e30;
+    for (int t = 0; t < 4; ++t) {
+        const double wg = (double)groups * (g.tab_off[t + 1] - g.tab_off[t]);
+        const double d = std::fabs(std::log(std::max(wg, 1.0) / target));
+        if (d < best_d - 1e-9) {
+            best_d = d;
+            best = t;
+        }
+    }
+    return best;
+}
+
 int fwd_pitch_of(int nx, int ny)
 {
     int p = std::max(nx, ny) + FWD_PADL + 4;
@@ -429,7 +489,8 @@ int fwd_pitch_of(int nx, int ny)
 
 int fwd_band_auto(int pitch, int K)
 {
-    int b = (int)(50 * 1024 / ((size_t)pitch * K * sizeof(float)));
+    // about 36 KB of LDS per work-group: four work-groups per CU
+    int b = (int)(36 * 1024 / ((size_t)pitch * K * sizeof(float)));
     return std::max(4, std::min(32, b & ~3));
 }
 
@@ -441,10 +502,20 @@ int adj_segw_of(int tile_y)
     return (w + 3) & ~3;
 }
 
+static int adj_lds_budget()
+{
+    static int kb = -1;
+    if (kb < 0) {
+        const char* e = getenv("GRAPTOR_ADJ_LDS_KB");
+        kb = e ? std::max(4, std::min(150, atoi(e))) : 36;
+    }
+    return kb;
+}
+
 int adj_achunk_of(int na, int segw, int K)
 {
     const size_t per_angle = (size_t)segw * K * sizeof(float) + sizeof(float4) + sizeof(int);
-    int a = (int)std::min<size_t>((size_t)na, (76 * 1024) / per_angle);
+    int a = (int)std::min<size_t>((size_t)na, (adj_lds_budget() * 1024) / per_angle);
     return std::max(a, 1);
 }
 
@@ -478,9 +549,15 @@ static int fwd_single_k(const ProbDev& Pin, const Geometry& g, int nslices, int 
 {
     ProbDev P = Pin;
     const int groups = (nslices + K - 1) / K;
-    // chunk table: fewest items per thread that still gives >= 1024 groups
-    int t = 0;
-    while (t < 3 && (long long)groups * (g.tab_off[t + 1] - g.tab_off[t]) < 1024) ++t;
+    // chunk table: items per thread (8, 4, 2, 1) whose grid is closest to
+    // three work-groups per CU (GRAPTOR_FWD_MI = 8, 4, 2 or 1 forces it)
+    int t = fwd_table_choice(g, groups);
+    static const int forced = [] {
+        const char* e = getenv("GRAPTOR_FWD_MI");
+        const int v = e ? atoi(e) : 0;
+        return v == 8 ? 0 : v == 4 ? 1 : v == 2 ? 2 : v == 1 ? 3 : -1;
+    }();
+    if (forced >= 0) t = forced;
     const int nchunks = g.tab_off[t + 1] - g.tab_off[t];
     const FwdChunk* chunks = g.chunks_dev + g.tab_off[t];
     P.fwd_pitch = fwd_pitch_of(P.nx, P.ny);
@@ -568,6 +645,43 @@ int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st)
     }
 }
 
+int launch_primal(const ProbDev& P, hipStream_t st)
+{
+    const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
+    const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
+    const long long blocks = (long long)tiles_x * tiles_y * P.nz * P.nch;
+    if (blocks > 0x7fffffff) {
+        set_error("primal step: grid too large");
+        return GR_ERR_INVALID;
+    }
+    dim3 grd((unsigned)blocks), blk(DUAL_BX * DUAL_BY);
+    if (P.dim == 2) {
+        if (P.tgv) hipLaunchKernelGGL((primal_single_kernel<2, true>), grd, blk, 0, st, P, tiles_x, tiles_y);
+        else hipLaunchKernelGGL((primal_single_kernel<2, false>), grd, blk, 0, st, P, tiles_x, tiles_y);
+    } else {
+        if (P.tgv) hipLaunchKernelGGL((primal_single_kernel<3, true>), grd, blk, 0, st, P, tiles_x, tiles_y);
+        else hipLaunchKernelGGL((primal_single_kernel<3, false>), grd, blk, 0, st, P, tiles_x, tiles_y);
+    }
+    GR_LAUNCH_CHECK("primal_single_kernel");
+    return GR_OK;
+}
+
+int launch_primal_batch(const ProbDev* probs, const DualItem* items, int nitems, int dim,
+                        bool tgv, hipStream_t st)
+{
+    if (nitems <= 0) return GR_OK;
+    dim3 grd((unsigned)nitems), blk(DUAL_BX * DUAL_BY);
+    if (dim == 2) {
+        if (tgv) hipLaunchKernelGGL((primal_batch_kernel<2, true>), grd, blk, 0, st, probs, items);
+        else hipLaunchKernelGGL((primal_batch_kernel<2, false>), grd, blk, 0, st, probs, items);
+    } else {
+        if (tgv) hipLaunchKernelGGL((primal_batch_kernel<3, true>), grd, blk, 0, st, probs, items);
+        else hipLaunchKernelGGL((primal_batch_kernel<3, false>), grd, blk, 0, st, probs, items);
+    }
+    GR_LAUNCH_CHECK("primal_batch_kernel");
+    return GR_OK;
+}
+
 // --------------------------------------------------------------------------
 // batch launchers
 // --------------------------------------------------------------------------
@@ -593,14 +707,15 @@ int launch_fwd_batch(const ProbDev* probs, const FwdItem* items, int nitems, int
 }
 
 int launch_adj_batch(const ProbDev* probs, const AdjItem* items, int nitems, int K, int dim,
-                     bool tgv, size_t lds, hipStream_t st)
+                     bool tgv, bool primal, size_t lds, hipStream_t st)
 {
     if (nitems <= 0) return GR_OK;
     dim3 grd((unsigned)nitems), blk(ADJ_THREADS);
 #define GR_AB(KK, DD, TT) \
-    hipLaunchKernelGGL((adj_batch_kernel<KK, DD, TT>), grd, blk, lds, st, probs, items)
+    hipLaunchKernelGGL((adj_batch_kernel<KK, true, DD, TT>), grd, blk, lds, st, probs, items)
 #define GR_AB_K(KK)                                          \
-    if (dim == 2) { if (tgv) GR_AB(KK, 2, true); else GR_AB(KK, 2, false); } \
+    if (!primal) hipLaunchKernelGGL((adj_batch_kernel<KK, false, 2, false>), grd, blk, lds, st, probs, items); \
+    else if (dim == 2) { if (tgv) GR_AB(KK, 2, true); else GR_AB(KK, 2, false); } \
     else { if (tgv) GR_AB(KK, 3, true); else GR_AB(KK, 3, false); }
     if (K == 4) { GR_AB_K(4) } else if (K == 2) { GR_AB_K(2) } else { GR_AB_K(1) }
 #undef GR_AB_K

@@ -275,6 +275,9 @@ __device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
     }
 }
 
+// Work order: x tiles fastest, then rows, then planes (plane-major order
+// re-reads fewer neighbour planes but spreads concurrent work-groups over
+// fewer HBM channels, and measured slower).
 template <int DIM, int CPL, bool TGV, int MAXC>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_single_kernel(ProbDev P, int tiles_x,
                                                                        int tiles_y)

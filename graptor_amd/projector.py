@@ -47,6 +47,7 @@ class RadonGeometry:
         _lib.check(lib.gr_geometry_create(ofs.ctypes.data, self.na, nx, ny, nd,
                                           ctypes.byref(h)), "geometry_create")
         self._h = h
+        self.fuse_primal = False
 
     @property
     def handle(self):
@@ -59,6 +60,13 @@ class RadonGeometry:
     def set_tuning(self, fwd_band=0, adj_tile_y=0):
         _lib.check(_lib.load().gr_geometry_set_tuning(self._h, int(fwd_band), int(adj_tile_y)),
                    "set_tuning")
+
+    def set_fusion(self, fuse_primal: bool):
+        """Run the primal step inside the backprojection kernel (True) or as
+        a separate kernel reading K^T lam from memory (False, default)."""
+        _lib.check(_lib.load().gr_geometry_set_fusion(self._h, 1 if fuse_primal else 0),
+                   "set_fusion")
+        self.fuse_primal = bool(fuse_primal)
 
     def get_tuning(self):
         a, b = ctypes.c_int(), ctypes.c_int()

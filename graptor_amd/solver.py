@@ -148,7 +148,9 @@ class TgvProblem:
         self.v = z(d, *img)
         self.w = z(m, *img) if tgv else None
         self.lam = torch.zeros_like(self.f)
-        self.ktl = None  # K^T lam stays inside the fused backprojection kernel
+        # K^T lam: needed unless the geometry fuses the primal step into the
+        # backprojection
+        self.ktl = None if getattr(self.geometry, "fuse_primal", False) else z(*img)
         self.ku = torch.zeros_like(self.f)
         self.mfac = torch.as_tensor(self.k.mfac.astype(np.float32), device=self.device)
         self.params = _lib.TgvParams(
@@ -182,8 +184,8 @@ class TgvProblem:
     def profile(self, n: int):
         """Run ``n`` iterations with HIP events around each kernel; returns the
         summed milliseconds of (dual step, forward projection + prox,
-        backprojection + primal step)."""
-        out = (ctypes.c_double * 3)()
+        backprojection [+ primal step when fused], primal step)."""
+        out = (ctypes.c_double * 4)()
         with torch.cuda.device(self.device):
             _lib.check(_lib.load().gr_tgv_profile(
                 self.geometry.handle, ctypes.byref(self.params), ctypes.byref(self.state),

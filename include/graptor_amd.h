@@ -56,6 +56,10 @@ int gr_geometry_destroy(gr_geometry *g);
 /* Tile shapes of the projector kernels (autotuner hook); 0 keeps a value. */
 int gr_geometry_set_tuning(gr_geometry *g, int fwd_band, int adj_tile_y);
 int gr_geometry_get_tuning(const gr_geometry *g, int *fwd_band, int *adj_tile_y);
+/* 1: run the primal step inside the backprojection kernel (K^T lam never
+ * stored); 0 (default): backprojection writes st->ktl and a separate
+ * primal kernel reads it. */
+int gr_geometry_set_fusion(gr_geometry *g, int fuse_primal);
 
 /* Replaces radon() 2D:544-550 / 3D:988-994 (kernel 2D:391-434):
  * sino[s][angle][bin] = R img[s][y][x] for s < nslices. */
@@ -106,22 +110,22 @@ typedef struct gr_tgv_state {
     const float *f, *mfac;
 } gr_tgv_state;
 
-/* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 on `stream`
- * as three kernels: dual step with the coupled projections; forward
- * projection fused with the KL/L2 prox; backprojection fused with the primal
- * step and the extrapolation (K^T lam never reaches memory, so st->ktl may
- * be NULL).  With ku_on_last,
+/* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 on `stream`:
+ * dual step with the coupled projections; forward projection fused with the
+ * KL/L2 prox; backprojection; primal step fused with the extrapolation
+ * (inside the backprojection kernel when the geometry's fusion is on, in
+ * which case st->ktl may be NULL).  With ku_on_last,
  * the last iteration also stores R(ubar) into ku, the sinogram the reference
  * returns (2D:840-841). */
 int gr_tgv_iterate(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, int ku_on_last,
                    void *stream);
 
-/* gr_tgv_iterate with a HIP event pair around each of the three kernels of
- * every iteration (measurement only; no counterpart in the reference).
- * ms_out[0..2] receive the summed kernel times in milliseconds of the dual
- * step, the forward projection + prox and the backprojection + primal step.
- * Synchronises the stream. */
+/* gr_tgv_iterate with a HIP event pair around each kernel of every
+ * iteration (measurement only; no counterpart in the reference).  ms_out[0..3]
+ * receive the summed kernel times in milliseconds of the dual step, the
+ * forward projection + prox, the backprojection (+ primal step when fused)
+ * and the primal step (0 when fused).  Synchronises the stream. */
 int gr_tgv_profile(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, double *ms_out, void *stream);
 
@@ -152,7 +156,7 @@ int gr_plan_iterate(gr_plan *plan, int n_iter, int ku_on_last, void *stream);
  * private stream and replayed on `stream` n_iter / block times; the
  * remaining iterations run as ordinary launches. */
 int gr_plan_iterate_graph(gr_plan *plan, int n_iter, int block, void *stream);
-/* Per-phase kernel times (ms_out[3], as gr_tgv_profile) over n_iter
+/* Per-phase kernel times (ms_out[4], as gr_tgv_profile) over n_iter
  * iterations.  Synchronises the stream. */
 int gr_plan_profile(gr_plan *plan, int n_iter, double *ms_out, void *stream);
 /* Re-reads every geometry's tile shapes (gr_geometry_set_tuning) into the
