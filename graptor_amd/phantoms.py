@@ -117,3 +117,57 @@ def ellipse_problem_2d(N_support=128, N_grid=183, na=90, lo=-70.0, hi=70.0, seed
     s = s + nrng.normal(0.0, noise * s.max(), s.shape)
     return (torch.as_tensor(s.astype(np.float32), device=device).view(1, 1, na, N_grid), angles,
             ph, geom)
+
+
+def multichannel_slice(rng, N, C, n_min=5, n_max=12):
+    """C-channel 2D slice: channel 0 a weighted sum of the C-1 element maps
+    (or one map when C == 1), the others the maps; each map holds
+    n_min..n_max random ellipses with values U[0.5, 1] inside a disc."""
+    yy, xx = np.mgrid[0:N, 0:N] - (N - 1) / 2.0
+    nmaps = max(C - 1, 1)
+    maps = np.zeros((nmaps, N, N))
+    for k in range(nmaps):
+        for _ in range(rng.integers(n_min, n_max + 1)):
+            cx, cy = rng.uniform(-0.3, 0.3, 2) * N
+            a, b = rng.uniform(0.03, 0.15, 2) * N
+            th = rng.uniform(0, np.pi)
+            xr = (xx - cx) * np.cos(th) + (yy - cy) * np.sin(th)
+            yr = -(xx - cx) * np.sin(th) + (yy - cy) * np.cos(th)
+            np.maximum(maps[k], rng.uniform(0.5, 1.0) * ((xr / a) ** 2 + (yr / b) ** 2 < 1.0),
+                       out=maps[k])
+    matrix = (xx ** 2 + yy ** 2 < (0.42 * N) ** 2).astype(np.float64)
+    if C == 1:
+        return (0.25 * matrix + maps[0])[None]
+    w = rng.uniform(0.5, 1.0, nmaps)
+    haadf = 0.25 * matrix + np.tensordot(w, maps, axes=1)
+    return np.concatenate([haadf[None], maps], axis=0)
+
+
+def batched_2d_problem_set(n=64, seed=3000, counts=(1e3, 20.0), device="cuda"):
+    """The 2D batch of uneven problems: for problem k (seed + k) the image
+    size N ~ U{96..384} (grid nd = N), angles ~ U{30..180} over [-70, 70)
+    degrees, channels ~ U{1..4}; channel 0 Poisson at ``counts[0]`` mean
+    counts per bin, the others at ``counts[1]``.  Returns a list of dicts
+    (f (C, 1, na, N) CUDA tensor, geometry, C, N, na, phantom)."""
+    out = []
+    for k in range(n):
+        rng = np.random.default_rng(seed + k)
+        N = int(rng.integers(96, 385))
+        na = int(rng.integers(30, 181))
+        C = int(rng.integers(1, 5))
+        ph = multichannel_slice(rng, N, C).astype(np.float32)
+        angles = tilt_angles(-70.0, 70.0, na, endpoint=False)
+        geom = RadonGeometry((N, N), angles, n_detectors=N)
+        img = torch.as_tensor(ph, device=device)
+        sino = torch.zeros((C, na, N), device=device)
+        radon(sino, img, geom)
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed + 100000 + k)
+        f = torch.empty_like(sino)
+        for c in range(C):
+            s = sino[c]
+            mean = s[s > 0].mean()
+            target = counts[0] if c == 0 else counts[1]
+            f[c] = torch.poisson(torch.clamp(s * (target / mean), min=0), generator=gen)
+        out.append(dict(f=f.view(C, 1, na, N), geometry=geom, C=C, N=N, na=na, phantom=ph))
+    return out
