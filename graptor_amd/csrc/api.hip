@@ -217,25 +217,31 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
     pl->dual.clear();
     pl->primal.clear();
     const int n = pl->nprob;
-    // forward chunk table: the items per thread whose batch grid is closest
-    // to three work-groups per CU
-    int t = 0;
-    double best_d = 1e30;
-    for (int tt = 0; tt < 4; ++tt) {
-        long long total = 0;
-        for (int i = 0; i < n; ++i) {
-            const Geometry& g = geoms[i]->g;
-            const int S = pl->probs[i].nz * pl->probs[i].nch;
-            const int K = slice_group(S);
-            total += (long long)((S + K - 1) / K) * (g.tab_off[tt + 1] - g.tab_off[tt]);
-        }
-        const double d = std::fabs(std::log(std::max((double)total, 1.0) / 768.0));
-        if (d < best_d - 1e-9) {
-            best_d = d;
-            t = tt;
+    // forward chunk tables: a common target of (bin, line) pairs per
+    // work-group (the batch's total over ~3 work-groups per CU); each problem
+    // takes the items-per-thread table whose chunks come closest to it
+    double total_work = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const ProbDev& P = pl->probs[i];
+        const int S = P.nz * P.nch, K = slice_group(S);
+        total_work += (double)((S + K - 1) / K) * P.na * P.nd * std::max(P.nx, P.ny);
+    }
+    const double target = std::max(1.0, total_work / 768.0);
+    std::vector<int> tsel(n, 0);
+    for (int i = 0; i < n; ++i) {
+        const ProbDev& P = pl->probs[i];
+        const Geometry& g = geoms[i]->g;
+        double best = 1e30;
+        for (int tt = 0; tt < 4; ++tt) {
+            const int nc = g.tab_off[tt + 1] - g.tab_off[tt];
+            const double per = (double)P.na * P.nd * std::max(P.nx, P.ny) / std::max(nc, 1);
+            const double d = std::fabs(std::log(per / target));
+            if (d < best - 1e-9) {
+                best = d;
+                tsel[i] = tt;
+            }
         }
     }
-    const int mi = 8 >> t;
     std::map<std::tuple<int, int>, size_t> fwd_idx;
     std::map<std::tuple<int, int, int, int>, size_t> adj_idx;
     std::map<std::tuple<int, int>, size_t> primal_idx;
@@ -246,6 +252,7 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
         const int S = P.nz * P.nch;
         const int K = slice_group(S);
         // forward items: slice groups x chunks (group-major)
+        const int t = tsel[i], mi = 8 >> t;
         auto fk = std::make_tuple(K, mi);
         if (!fwd_idx.count(fk)) {
             fwd_idx[fk] = pl->fwd.size();
@@ -293,6 +300,15 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
         for (int z = 0; z < P.nz; ++z)
             for (int y0 = 0; y0 < P.ny; y0 += DUAL_BY)
                 for (int x0 = 0; x0 < P.nx; x0 += DUAL_BX) dl.items.push_back(DualItem{i, x0, y0, z});
+    }
+    // largest chunks first, so the smallest fill the tail of the launch
+    for (auto& l : pl->fwd) {
+        auto work = [&](const FwdItem& it) {
+            const ProbDev& P = pl->probs[it.prob];
+            return (double)it.na * it.nb * (it.cls == 0 ? P.nx : P.ny);
+        };
+        std::stable_sort(l.items.begin(), l.items.end(),
+                         [&](const FwdItem& a, const FwdItem& b) { return work(a) > work(b); });
     }
     for (auto& l : pl->fwd)
         if (l.lds > 160 * 1024) {

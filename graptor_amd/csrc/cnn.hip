@@ -52,8 +52,8 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 // Convolution forward as implicit GEMM: M = pixels (32 consecutive x of one
 // row per wave tile), N = output channels (32 per wave tile), K = taps x ci.
 // A work-group of 4 waves covers 64 pixels x 64 channels (2 x 2 wave tiles).
-// Requires cin % 16 == 0, cout % 32 == 0, width % 32 == 0, channel offsets
-// and strides multiples of 8.
+// Requires cin % 16 == 0, cout % 32 == 0, channel offsets and strides
+// multiples of 8; rows of any width (partial wave tiles are masked).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void conv_fwd_kernel(
     const uint16_t* __restrict__ x, int ldx, int cx0, const uint16_t* __restrict__ w,
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(
     for (int t = 0; t < taps; ++t) {
         const int dy = taps == 9 ? t / 3 - 1 : 0, dx = taps == 9 ? t % 3 - 1 : 0;
         const int yy = row + dy, xx = px + dx;
-        const bool inb = yy >= 0 && yy < H && xx >= 0 && xx < W;
+        const bool inb = px < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
         const uint16_t* xa =
             x + ((size_t)(img * H + (inb ? yy : 0)) * W + (inb ? xx : 0)) * ldx + cx0 + 8 * h;
         const uint16_t* wb = w + ((size_t)co * taps + t) * cin + 8 * h;
@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256) void conv_fwd_kernel(
     for (int r = 0; r < 16; ++r) {
         const int m = 8 * (r >> 2) + 4 * h + (r & 3);
         const int xo = xbase + m;
+        if (xo >= W) continue;
         float v = acc[r] + bv;
         if (relu) v = v > 0.f ? v : 0.f;
         const size_t pix = (size_t)(img * H + row) * W + xo;
@@ -379,9 +380,9 @@ int gr_cnn_conv_fwd(const void* x, int ldx, int cx0, const void* w, const float*
                     int ldy, int cy0, int n, int H, int W, int cin, int cout, int taps, int relu,
                     int out_f32, void* stream)
 {
-    if (!x || !w || !y || n <= 0 || H <= 0 || W % 32 || cin % 16 || cout % 32 ||
+    if (!x || !w || !y || n <= 0 || H <= 0 || W <= 0 || cin % 16 || cout % 32 ||
         (taps != 1 && taps != 9) || ldx % 8 || cx0 % 8 || cx0 + cin > ldx || cy0 + cout > ldy) {
-        set_error("cnn_conv_fwd: unsupported shape (cin %% 16, cout %% 32, W %% 32, 8-aligned ld)");
+        set_error("cnn_conv_fwd: unsupported shape (cin %% 16, cout %% 32, 8-aligned ld)");
         return GR_ERR_INVALID;
     }
     const int tiles_x = (W + 63) / 64;

@@ -215,7 +215,9 @@ __global__ __launch_bounds__(FWD_THREADS) void fwd_batch_kernel(const ProbDev* _
                                                                 int mode)
 {
     extern __shared__ float sh[];
-    const FwdItem I = items[xcd_work_id(blockIdx.x, gridDim.x)];
+    // items are sorted largest first; plain block order deals them
+    // round-robin over the XCDs, which balances the load
+    const FwdItem I = items[blockIdx.x];
     const ProbDev P = probs[I.prob];
     fwd_body<K, MI>(P, I, mode, sh);
 }
