@@ -31,6 +31,9 @@ EXPORTS = (
     "gr_radon", "gr_radon_ad", "gr_normest", "gr_make_feasible", "gr_tgv_iterate",
     "gr_tgv_energy", "gr_tgv_profile", "gr_plan_create", "gr_plan_destroy",
     "gr_plan_iterate", "gr_plan_iterate_graph", "gr_plan_profile", "gr_plan_retune",
+    "gr_cnn_conv_fwd", "gr_cnn_conv_wgrad", "gr_cnn_bias_grad", "gr_cnn_weight_flip",
+    "gr_cnn_relu_bwd", "gr_cnn_pool", "gr_cnn_upsample", "gr_cnn_pack_bf16", "gr_cnn_mse_grad",
+    "gr_cnn_adam",
 )
 
 
@@ -76,6 +79,7 @@ def load(path: str | None = None):
                 "`python -m graptor_amd.build`")
         lib = ctypes.CDLL(p)
         V, I, D, F = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float
+        S = ctypes.c_size_t
         PD = ctypes.POINTER(ctypes.c_double)
         PI = ctypes.POINTER(ctypes.c_int)
         sig = {
@@ -104,6 +108,16 @@ def load(path: str | None = None):
             "gr_plan_iterate_graph": ([V, I, I, V], I),
             "gr_plan_profile": ([V, I, V, V], I),
             "gr_plan_retune": ([V, V], I),
+            "gr_cnn_conv_fwd": ([V, I, I, V, V, V, I, I, I, I, I, I, I, I, I, I, V], I),
+            "gr_cnn_conv_wgrad": ([V, I, I, V, I, I, V, I, I, I, I, I, I, I, V], I),
+            "gr_cnn_bias_grad": ([V, I, I, S, I, V, V], I),
+            "gr_cnn_weight_flip": ([V, V, I, I, I, V], I),
+            "gr_cnn_relu_bwd": ([V, I, I, V, I, I, S, I, V], I),
+            "gr_cnn_pool": ([V, I, I, V, I, I, I, I, I, I, I, I, V], I),
+            "gr_cnn_upsample": ([V, I, I, V, I, I, I, I, I, I, I, V], I),
+            "gr_cnn_pack_bf16": ([V, I, V, I, I, S, F, V], I),
+            "gr_cnn_mse_grad": ([V, I, V, I, S, V, I, F, V, V], I),
+            "gr_cnn_adam": ([V, V, V, V, V, S, F, F, F, F, I, F, V], I),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

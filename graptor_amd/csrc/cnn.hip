@@ -374,6 +374,9 @@ static hipStream_t cnn_stream(void* s) { return reinterpret_cast<hipStream_t>(s)
 
 static unsigned nblk(size_t n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
+// a channel slice [c0, c0 + c) of a buffer with ld channels per pixel
+static bool slice_ok(int ld, int c0, int c) { return c > 0 && c0 >= 0 && c0 + c <= ld; }
+
 extern "C" {
 
 int gr_cnn_conv_fwd(const void* x, int ldx, int cx0, const void* w, const float* bias, void* y,
@@ -400,7 +403,8 @@ int gr_cnn_conv_wgrad(const void* x, int ldx, int cx0, const void* dy, int ldd, 
                       int rows_per_chunk, void* stream)
 {
     if (!x || !dy || !dw || n <= 0 || H <= 0 || W <= 0 || cin % 8 || cout % 8 || ldx % 8 ||
-        cx0 % 8 || ldd % 8 || cd0 % 8 || (taps != 1 && taps != 9) || rows_per_chunk <= 0) {
+        cx0 % 8 || ldd % 8 || cd0 % 8 || cx0 + cin > ldx || cd0 + cout > ldd ||
+        (taps != 1 && taps != 9) || rows_per_chunk <= 0) {
         set_error("cnn_conv_wgrad: invalid arguments");
         return GR_ERR_INVALID;
     }
@@ -417,7 +421,7 @@ int gr_cnn_conv_wgrad(const void* x, int ldx, int cx0, const void* dy, int ldd, 
 int gr_cnn_bias_grad(const void* dy, int ldd, int cd0, size_t npix, int cout, float* db,
                      void* stream)
 {
-    if (!dy || !db || cout <= 0 || cout > 1024) {
+    if (!dy || !db || npix == 0 || cout <= 0 || cout > 1024 || !slice_ok(ldd, cd0, cout)) {
         set_error("cnn_bias_grad: invalid arguments");
         return GR_ERR_INVALID;
     }
@@ -430,6 +434,10 @@ int gr_cnn_bias_grad(const void* dy, int ldd, int cd0, size_t npix, int cout, fl
 
 int gr_cnn_weight_flip(const void* w, void* wt, int cin, int cout, int taps, void* stream)
 {
+    if (!w || !wt || cin <= 0 || cout <= 0 || (taps != 1 && taps != 9)) {
+        set_error("cnn_weight_flip: invalid arguments");
+        return GR_ERR_INVALID;
+    }
     const size_t total = (size_t)cin * cout * taps;
     hipLaunchKernelGGL(weight_flip_kernel, dim3(nblk(total, 256)), dim3(256), 0,
                        cnn_stream(stream), (const uint16_t*)w, (uint16_t*)wt, cin, cout, taps);
@@ -440,6 +448,10 @@ int gr_cnn_weight_flip(const void* w, void* wt, int cin, int cout, int taps, voi
 int gr_cnn_relu_bwd(void* dy, int ldd, int cd0, const void* y, int ldy, int cy0, size_t npix,
                     int c, void* stream)
 {
+    if (!dy || !y || npix == 0 || !slice_ok(ldd, cd0, c) || !slice_ok(ldy, cy0, c)) {
+        set_error("cnn_relu_bwd: invalid arguments");
+        return GR_ERR_INVALID;
+    }
     hipLaunchKernelGGL(relu_bwd_kernel, dim3(nblk(npix * c, 256)), dim3(256), 0,
                        cnn_stream(stream), (uint16_t*)dy, ldd, cd0, (const uint16_t*)y, ldy, cy0,
                        npix, c);
@@ -450,8 +462,9 @@ int gr_cnn_relu_bwd(void* dy, int ldd, int cd0, const void* y, int ldy, int cy0,
 int gr_cnn_pool(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, int n, int H, int W,
                 int c, int backward, int accumulate, void* stream)
 {
-    if (H % 2 || W % 2) {
-        set_error("cnn_pool: even sizes required");
+    if (!x || !y || n <= 0 || H <= 0 || W <= 0 || H % 2 || W % 2 || !slice_ok(ldx, cx0, c) ||
+        !slice_ok(ldy, cy0, c)) {
+        set_error("cnn_pool: invalid arguments (even sizes, channel slices within ld)");
         return GR_ERR_INVALID;
     }
     if (!backward)
@@ -469,6 +482,11 @@ int gr_cnn_pool(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, int 
 int gr_cnn_upsample(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, int n, int H,
                     int W, int c, int backward, void* stream)
 {
+    if (!x || !y || n <= 0 || H <= 0 || W <= 0 || !slice_ok(ldx, cx0, c) ||
+        !slice_ok(ldy, cy0, c)) {
+        set_error("cnn_upsample: invalid arguments");
+        return GR_ERR_INVALID;
+    }
     if (!backward)
         hipLaunchKernelGGL(up_fwd_kernel, dim3(nblk((size_t)n * 4 * H * W * c, 256)), dim3(256),
                            0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
@@ -484,6 +502,10 @@ int gr_cnn_upsample(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, 
 int gr_cnn_pack_bf16(const float* x, int c_in, void* y, int ldy, int cy0, size_t npix,
                      float scale, void* stream)
 {
+    if (!x || !y || npix == 0 || !slice_ok(ldy, cy0, c_in)) {
+        set_error("cnn_pack_bf16: invalid arguments");
+        return GR_ERR_INVALID;
+    }
     hipLaunchKernelGGL(pack_bf16_kernel, dim3(nblk(npix * c_in, 256)), dim3(256), 0,
                        cnn_stream(stream), x, c_in, (uint16_t*)y, ldy, cy0, npix, scale);
     GR_LAUNCH_CHECK("pack_bf16_kernel");
@@ -493,6 +515,11 @@ int gr_cnn_pack_bf16(const float* x, int c_in, void* y, int ldy, int cy0, size_t
 int gr_cnn_mse_grad(const float* pred, int ldp, const float* target, int c, size_t npix, void* g,
                     int ldg, float gscale, float* loss, void* stream)
 {
+    if (!pred || !target || !g || !loss || npix == 0 || !slice_ok(ldp, 0, c) ||
+        !slice_ok(ldg, 0, c)) {
+        set_error("cnn_mse_grad: invalid arguments");
+        return GR_ERR_INVALID;
+    }
     hipLaunchKernelGGL(mse_grad_kernel, dim3(nblk(npix * c, 256)), dim3(256), 0,
                        cnn_stream(stream), pred, ldp, target, c, npix, (uint16_t*)g, ldg, gscale,
                        loss);
@@ -503,6 +530,10 @@ int gr_cnn_mse_grad(const float* pred, int ldp, const float* target, int c, size
 int gr_cnn_adam(float* p, const float* g, float* m, float* v, void* pb, size_t n, float lr,
                 float b1, float b2, float eps, int step, float wd, void* stream)
 {
+    if (!p || !g || !m || !v || n == 0 || step <= 0) {
+        set_error("cnn_adam: invalid arguments");
+        return GR_ERR_INVALID;
+    }
     const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
     hipLaunchKernelGGL(adam_kernel, dim3(nblk(n, 256)), dim3(256), 0, cnn_stream(stream), p, g, m,
                        v, (uint16_t*)pb, n, lr, b1, b2, eps, c1, c2, wd);

@@ -119,28 +119,94 @@ def ellipse_problem_2d(N_support=128, N_grid=183, na=90, lo=-70.0, hi=70.0, seed
             ph, geom)
 
 
-def multichannel_slice(rng, N, C, n_min=5, n_max=12):
-    """C-channel 2D slice: channel 0 a weighted sum of the C-1 element maps
-    (or one map when C == 1), the others the maps; each map holds
-    n_min..n_max random ellipses with values U[0.5, 1] inside a disc."""
-    yy, xx = np.mgrid[0:N, 0:N] - (N - 1) / 2.0
+def multichannel_params(rng, N, C, n_min=5, n_max=12):
+    """Random draws of one C-channel slice: per element map a list of
+    ellipses (cx, cy, a, b, theta, value) in pixels about the centre, and the
+    map weights of channel 0."""
     nmaps = max(C - 1, 1)
-    maps = np.zeros((nmaps, N, N))
-    for k in range(nmaps):
+    maps = []
+    for _ in range(nmaps):
+        ell = []
         for _ in range(rng.integers(n_min, n_max + 1)):
             cx, cy = rng.uniform(-0.3, 0.3, 2) * N
             a, b = rng.uniform(0.03, 0.15, 2) * N
             th = rng.uniform(0, np.pi)
+            ell.append((cx, cy, a, b, th, rng.uniform(0.5, 1.0)))
+        maps.append(ell)
+    w = rng.uniform(0.5, 1.0, nmaps) if C > 1 else None
+    return dict(N=N, C=C, maps=maps, w=w)
+
+
+def multichannel_slice(rng, N, C, n_min=5, n_max=12):
+    """C-channel 2D slice: channel 0 a weighted sum of the C-1 element maps
+    (or one map when C == 1), the others the maps; each map holds
+    n_min..n_max random ellipses with values U[0.5, 1] inside a disc."""
+    prm = multichannel_params(rng, N, C, n_min, n_max)
+    yy, xx = np.mgrid[0:N, 0:N] - (N - 1) / 2.0
+    maps = np.zeros((len(prm["maps"]), N, N))
+    for k, ell in enumerate(prm["maps"]):
+        for cx, cy, a, b, th, val in ell:
             xr = (xx - cx) * np.cos(th) + (yy - cy) * np.sin(th)
             yr = -(xx - cx) * np.sin(th) + (yy - cy) * np.cos(th)
-            np.maximum(maps[k], rng.uniform(0.5, 1.0) * ((xr / a) ** 2 + (yr / b) ** 2 < 1.0),
-                       out=maps[k])
+            np.maximum(maps[k], val * ((xr / a) ** 2 + (yr / b) ** 2 < 1.0), out=maps[k])
     matrix = (xx ** 2 + yy ** 2 < (0.42 * N) ** 2).astype(np.float64)
     if C == 1:
         return (0.25 * matrix + maps[0])[None]
-    w = rng.uniform(0.5, 1.0, nmaps)
-    haadf = 0.25 * matrix + np.tensordot(w, maps, axes=1)
+    haadf = 0.25 * matrix + np.tensordot(prm["w"], maps, axes=1)
     return np.concatenate([haadf[None], maps], axis=0)
+
+
+def multichannel_slices(seeds, N, C, device="cuda"):
+    """The slices multichannel_slice(default_rng(seed), N, C) for many seeds
+    at once, rasterised on the GPU in float64 (same draws and the same inside
+    tests).  Returns a float32 tensor (len(seeds), C, N, N)."""
+    prms = [multichannel_params(np.random.default_rng(int(s)), N, C) for s in seeds]
+    B, nmaps = len(prms), max(C - 1, 1)
+    E = max(len(ell) for p in prms for ell in p["maps"])
+    P = np.zeros((B, nmaps, E, 6))
+    for i, p in enumerate(prms):
+        for k, ell in enumerate(p["maps"]):
+            if ell:
+                P[i, k, :len(ell)] = np.asarray(ell)
+    Pt = torch.as_tensor(P, device=device)
+    c = (torch.arange(N, dtype=torch.float64, device=device) - (N - 1) / 2.0)
+    yy, xx = c.view(1, 1, N, 1), c.view(1, 1, 1, N)
+    maps = torch.zeros((B, nmaps, N, N), dtype=torch.float64, device=device)
+    for e in range(E):
+        cx, cy, a, b, th, val = (Pt[:, :, e, j].view(B, nmaps, 1, 1) for j in range(6))
+        valid = a > 0
+        a = torch.where(valid, a, torch.ones_like(a))
+        b = torch.where(valid, b, torch.ones_like(b))
+        ct, st = torch.cos(th), torch.sin(th)
+        xr = (xx - cx) * ct + (yy - cy) * st
+        yr = -(xx - cx) * st + (yy - cy) * ct
+        inside = ((xr / a) ** 2 + (yr / b) ** 2 < 1.0) & valid
+        maps = torch.maximum(maps, val * inside)
+    matrix = ((xx ** 2 + yy ** 2) < (0.42 * N) ** 2).to(torch.float64).view(1, N, N)
+    if C == 1:
+        return (0.25 * matrix + maps[:, 0]).unsqueeze(1).float()
+    w = torch.as_tensor(np.stack([p["w"] for p in prms]), device=device)  # (B, nmaps)
+    haadf = 0.25 * matrix + (w.view(B, nmaps, 1, 1) * maps).sum(dim=1)
+    return torch.cat([haadf.unsqueeze(1), maps], dim=1).float()
+
+
+def poisson_sinograms(img, geom, counts=(1e3, 20.0), seed=0):
+    """Poisson data of a (B, C, N, N) image stack: channel 0 scaled to
+    ``counts[0]`` mean counts over the bins its projection reaches, the other
+    channels to ``counts[1]``; one torch generator seeded with ``seed``.
+    Returns (B, C, na, nd) float32."""
+    B, C = img.shape[:2]
+    sino = torch.zeros((B * C, geom.na, geom.nd), device=img.device)
+    radon(sino, img.reshape(B * C, *img.shape[2:]).contiguous(), geom)
+    sino = sino.view(B, C, geom.na, geom.nd)
+    pos = (sino > 0).to(sino.dtype)
+    mean = (sino * pos).sum(dim=(2, 3)) / pos.sum(dim=(2, 3)).clamp(min=1)
+    target = torch.full((B, C), float(counts[1]), device=img.device)
+    target[:, 0] = float(counts[0])
+    lam = torch.clamp(sino * (target / mean).view(B, C, 1, 1), min=0)
+    gen = torch.Generator(device=img.device)
+    gen.manual_seed(int(seed))
+    return torch.poisson(lam, generator=gen)
 
 
 def batched_2d_problem_set(n=64, seed=3000, counts=(1e3, 20.0), device="cuda"):

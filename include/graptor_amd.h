@@ -18,6 +18,8 @@
 #ifndef GRAPTOR_AMD_H
 #define GRAPTOR_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -162,6 +164,56 @@ int gr_plan_profile(gr_plan *plan, int n_iter, double *ms_out, void *stream);
 /* Re-reads every geometry's tile shapes (gr_geometry_set_tuning) into the
  * plan; used by the tile autotuner.  Drops a captured graph. */
 int gr_plan_retune(gr_plan *plan, const gr_geometry *const *geoms);
+
+/* ------------------------------------------------------------------------
+ * Learned warm start: BF16 MFMA kernels of a small U-Net (no counterpart in
+ * the reference, whose solver always starts from u = 0, 2D:730 / 3D:1161).
+ *
+ * Activations are NHWC bf16 with a per-pixel stride `ld*` (channels
+ * allocated) and a channel offset `c*0`, so a tensor may be a channel slice
+ * of a wider buffer.  Weights are W[co][tap][ci] (ci fastest), tap = 3*ky+kx
+ * for 3x3 convolutions with zero padding 1, or one tap for 1x1.  Products run
+ * on v_mfma_f32_32x32x16_bf16 with FP32 accumulation.
+ * ------------------------------------------------------------------------ */
+
+/* y[.., cy0+co] = act(bias[co] + sum_{tap,ci} W[co][tap][ci] x[shifted, cx0+ci]);
+ * cin % 16 == 0, cout % 32 == 0, ldx/cx0 multiples of 8; bias may be NULL;
+ * relu != 0 applies max(0, .); out_f32 != 0 writes float instead of bf16.
+ * The data gradient is this call with gr_cnn_weight_flip'ed weights. */
+int gr_cnn_conv_fwd(const void *x, int ldx, int cx0, const void *w, const float *bias, void *y,
+                    int ldy, int cy0, int n, int H, int W, int cin, int cout, int taps, int relu,
+                    int out_f32, void *stream);
+/* dw[co][tap][ci] += sum_pixels dy[p][co] x[p shifted by tap][ci] (FP32, atomic
+ * split-K over chunks of rows_per_chunk image rows). */
+int gr_cnn_conv_wgrad(const void *x, int ldx, int cx0, const void *dy, int ldd, int cd0,
+                      float *dw, int n, int H, int W, int cin, int cout, int taps,
+                      int rows_per_chunk, void *stream);
+/* db[co] += sum_pixels dy[p][co] */
+int gr_cnn_bias_grad(const void *dy, int ldd, int cd0, size_t npix, int cout, float *db,
+                     void *stream);
+/* wt[ci][taps-1-tap][co] = w[co][tap][ci] (bf16) */
+int gr_cnn_weight_flip(const void *w, void *wt, int cin, int cout, int taps, void *stream);
+/* dy *= (y > 0), in place */
+int gr_cnn_relu_bwd(void *dy, int ldd, int cd0, const void *y, int ldy, int cy0, size_t npix,
+                    int c, void *stream);
+/* backward == 0: y = 2x2 average pool of x (H, W of x).  backward != 0: y (fine,
+ * H x W) = 1/4 x (coarse) broadcast, added to y when accumulate != 0. */
+int gr_cnn_pool(const void *x, int ldx, int cx0, void *y, int ldy, int cy0, int n, int H, int W,
+                int c, int backward, int accumulate, void *stream);
+/* backward == 0: y = nearest 2x upsample of x (H, W of x).  backward != 0:
+ * y (coarse, H x W) = sum of x (fine) over each 2x2 block. */
+int gr_cnn_upsample(const void *x, int ldx, int cx0, void *y, int ldy, int cy0, int n, int H,
+                    int W, int c, int backward, void *stream);
+/* y[p][cy0+ch] = bf16(scale * x[p][ch]) for dense float x with c_in channels */
+int gr_cnn_pack_bf16(const float *x, int c_in, void *y, int ldy, int cy0, size_t npix,
+                     float scale, void *stream);
+/* g[p][ch] = bf16(gscale (pred[p][ch] - target[p][ch])); *loss += sum of squares */
+int gr_cnn_mse_grad(const float *pred, int ldp, const float *target, int c, size_t npix, void *g,
+                    int ldg, float gscale, float *loss, void *stream);
+/* Adam step (bias-corrected; wd adds wd * p to the gradient) on FP32 master
+ * weights; refreshes the bf16 copy pb when non-NULL. */
+int gr_cnn_adam(float *p, const float *g, float *m, float *v, void *pb, size_t n, float lr,
+                float b1, float b2, float eps, int step, float wd, void *stream);
 
 #ifdef __cplusplus
 }
