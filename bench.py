@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 ALPHA = (4.0, 1.0)
 MU = (0.2, 5e-4, 4e-3, 5e-4)
 KERNELS = ("dual_single_kernel", "fwd_single_kernel", "adj_single_kernel",
-           "primal_single_kernel")
+           "primal_single_kernel", "pd_fused_kernel")
 
 
 def kernel_bytes(prob):
@@ -46,7 +46,21 @@ def kernel_bytes(prob):
     adj = B * 4 + V * 4
     # primal step: read u, K^T lam, v, w, p; write u, ubar, p, pbar
     primal = V * 4 * (2 + d + 2 + ((m + 3 * d) if tgv else 0))
-    return dict(zip(KERNELS, (dual, fwd, adj, primal)))
+    # primal step + next dual step in one kernel: read u, K^T lam, v, w, p;
+    # write u, ubar, p, v, w (pbar is formed in registers, not stored)
+    fused = V * 4 * (2 + d + (m + d if tgv else 0) + 2 + d + (m + d if tgv else 0))
+    return dict(zip(KERNELS, (dual, fwd, adj, primal, fused)))
+
+
+def launch_counts(n, fused):
+    """Launches of (dual, fwd, adj, primal, fused) kernels in n iterations of
+    gr_tgv_iterate: with the fused form, one unfused iteration when n - 1 is
+    odd, then a standalone dual, n - 1 - that fused steps and a standalone
+    primal."""
+    if not fused or n < 2:
+        return (n, n, n, n, 0)
+    plain = (n - 1) % 2
+    return (plain + 1, n, n, plain + 1, n - plain - 1)
 
 
 def cpu_baseline(f_host, angles, L, iters):
@@ -78,7 +92,7 @@ def cpu_baseline(f_host, angles, L, iters):
     threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     vox = C * Nz * nd * nd
     return dict(value=vox / per_iter, unit="voxel-iters/s", cores=threads, kind="port",
-                sample=f"same 256x256x64x4 volume, {iters} iterations timed as the difference "
+                sample=f"same {nd}x{nd}x{Nz}x{C} volume, {iters} iterations timed as the difference "
                        f"of a {1 + iters}- and a 1-iteration run (allocation excluded), "
                        f"gcc -O3 {arch} OpenMP",
                 seconds_per_iter=per_iter)
@@ -105,7 +119,7 @@ def batched_main(args):
         mu = [0.02] + [0.5] * (sp["C"] - 1)
         probs.append(TgvProblem(sp["f"], sp["geometry"], ALPHA, mu[:sp["C"]], dim=2,
                                 coupling=cpl, discrepancy="KL", normest_mode="power",
-                                device=dev))
+                                device=dev, fused=False))
     plan = BatchPlan(probs)
     tune = autotune(probs, iters=3, plan=plan)
     print(f"# autotune: {tune}", file=sys.stderr, flush=True)
@@ -210,7 +224,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--workload", default="coupled3d",
-                    choices=["coupled3d", "batched2d", "warmstart"])
+                    choices=["coupled3d", "large3d", "batched2d", "warmstart"])
+    ap.add_argument("--cap-iters", type=int, default=2000,
+                    help="large3d: iteration cap of the run to a relative-change tolerance")
+    ap.add_argument("--change-tol", type=float, default=1e-3)
     ap.add_argument("--train-pairs", type=int, default=8192)
     ap.add_argument("--train-steps", type=int, default=6000)
     ap.add_argument("--eval-slices", type=int, default=256)
@@ -244,9 +261,13 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    f, angles, phantom, geom = coupled_granule_problem(Nz=args.nz, device=dev)
+    large = args.workload == "large3d"
+    N, nz, na = (512, 256, 90) if large else (256, args.nz, 60)
+    f, angles, phantom, geom = coupled_granule_problem(N=N, Nz=nz, na=na, device=dev)
+    del phantom
     prob = TgvProblem(f, geom, ALPHA, MU, dim=3, coupling="frobenius", discrepancy="KL",
                       normest_mode="power", device=dev)
+    del f
     print(f"# problem ready: {prob.C}x{prob.Nz}x{prob.ny}x{prob.nx}, na={prob.na}, "
           f"L={prob.k.L:.6f}", file=sys.stderr, flush=True)
     prob.iterate(args.warmup)
@@ -268,10 +289,11 @@ def main():
 
     # per-kernel durations: HIP events around every launch of a profiled pass
     ms = prob.profile(args.profile_iters)
-    avg = [m / args.profile_iters for m in ms]
+    counts = launch_counts(args.profile_iters, prob.u2 is not None)
     nbytes = kernel_bytes(prob)
-    kern = {k: dict(avg_ms=a, bytes=nbytes[k], gbs=nbytes[k] / (a * 1e-3) / 1e9)
-            for k, a in zip(KERNELS, avg)}
+    kern = {k: dict(avg_ms=m / c, launches=c, bytes=nbytes[k],
+                    gbs=nbytes[k] / (m / c * 1e-3) / 1e9)
+            for k, m, c in zip(KERNELS, ms, counts) if c > 0}
     dom = max(kern, key=lambda k: kern[k]["avg_ms"])
     achieved = kern[dom]["gbs"]
     print("# kernels: " + json.dumps(kern), file=sys.stderr, flush=True)
@@ -287,15 +309,42 @@ def main():
         del src, dst
 
     cpu = None
-    if not args.no_cpu_baseline:
+    # the C restatement keeps the reference's padded layouts (~0.4 KB per
+    # voxel-channel): the large volume would need ~100 GB of host memory
+    if not args.no_cpu_baseline and not large:
         f_host = prob.f.cpu().numpy()
         cpu = cpu_baseline(f_host, geom.angles, prob.k.L, args.cpu_iters)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
         cpu.pop("seconds_per_iter", None)
 
+    converge = None
+    if large and args.cap_iters > 0:
+        # run from zero to the iteration cap, stopping when the relative change
+        # of u over 50 iterations drops to change_tol (checks not timed)
+        run = TgvProblem(prob.f, geom, ALPHA, MU, dim=3, coupling="frobenius",
+                         discrepancy="KL", L=prob.k.L, feasible=False, device=dev)
+        del prob.u, prob.ubar, prob.p, prob.pbar, prob.v, prob.w, prob.ktl
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        it, ms, change, prev = 0, 0.0, None, run.u.clone()
+        while it < args.cap_iters:
+            e0.record()
+            run.iterate(50)
+            e1.record()
+            e1.synchronize()
+            ms += e0.elapsed_time(e1)
+            it += 50
+            change = float((run.u - prev).norm() / run.u.norm().clamp(min=1e-30))
+            prev.copy_(run.u)
+            if change <= args.change_tol:
+                break
+        converge = {"iterations": it, "seconds": ms / 1e3, "rel_change_per_50": change,
+                    "tolerance": args.change_tol, "cap": args.cap_iters,
+                    "memory_GiB": torch.cuda.max_memory_allocated(dev) / 2 ** 30}
+        print(f"# converge: {converge}", file=sys.stderr, flush=True)
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "coupled3d_traffic.json")
-    if args.nz == 64 and os.path.exists(tpath):
+    if not large and args.nz == 64 and os.path.exists(tpath):
         # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
         # (scaled by a 1 GiB copy) for this workload
         with open(tpath) as fh:
@@ -316,9 +365,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded granule phantom, Poisson counts)",
-        "config": {"workload": "3D Frobenius-coupled 4-channel TGV-KL, 256x256x%d slices, "
-                               "60 tilts in [-75,75] deg, 256 bins" % args.nz,
-                   "volume": [4, args.nz, 256, 256], "angles": 60, "bins": 256,
+        "config": {"workload": "3D Frobenius-coupled 4-channel TGV-KL, %dx%dx%d slices, "
+                               "%d tilts in [-75,75] deg, %d bins" % (N, N, nz, na, N),
+                   "volume": [4, nz, N, N], "angles": na, "bins": N,
                    "alpha": list(ALPHA), "mu": list(MU), "iterations_timed": args.steps},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -328,6 +377,8 @@ def main():
         "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
         "cpu_baseline": cpu,
     }
+    if converge is not None:
+        line["convergence"] = converge
     print(json.dumps(line), flush=True)
 
 

@@ -59,7 +59,8 @@ class TgvParams(ctypes.Structure):
 
 class TgvState(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
-                ("u", "ubar", "p", "pbar", "v", "w", "lam", "ktl", "ku", "f", "mfac")]
+                ("u", "ubar", "p", "pbar", "v", "w", "lam", "ktl", "ku", "f", "mfac",
+                 "u2", "v2", "w2")]
 
 
 _lib = None

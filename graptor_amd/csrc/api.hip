@@ -125,40 +125,91 @@ ProbDev make_prob(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
     return P;
 }
 
+// One launch with an optional HIP event pair accumulated into *acc.
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on = false;
+    hipStream_t st;
+    Timer(bool enable, hipStream_t s) : on(enable), st(s)
+    {
+        if (on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+        }
+    }
+    ~Timer()
+    {
+        if (on) {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+    template <class F>
+    int run(double* acc, F&& f)
+    {
+        if (on) (void)hipEventRecord(a, st);
+        int rc = f();
+        if (on && rc == GR_OK) {
+            (void)hipEventRecord(b, st);
+            hipError_t e = hipEventSynchronize(b);
+            if (e != hipSuccess) return hip_status(e, "tgv_profile");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            *acc += ms;
+        }
+        return rc;
+    }
+};
+
+// Iterations of one problem.  Unfused: dual, forward + prox, backprojection,
+// primal per iteration.  With the alternate buffers of the fused form (and
+// no primal step inside the backprojection) the primal step of iteration k
+// and the dual step of k + 1 run as one kernel:
+//   dual_0, fwd_0, adj_0, [fused_k, fwd_k+1, adj_k+1] for k < n-1, primal_n-1
+// alternating the state between the caller's arrays and the alternates; an
+// unfused first iteration makes the number of fused steps even, so the state
+// ends in the caller's arrays.  Both forms compute the same values.
+// ms_out (optional, 5 entries): summed ms of dual, forward + prox,
+// backprojection, primal and fused launches.
 int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_on_last,
-                   hipStream_t st, double* ms_out)
+                   hipStream_t st, double* ms_out, const gr_tgv_state* s)
 {
     const int nsl = P.nz * P.nch;
     const int pmode = P.disc == GR_DISC_KL ? FWD_KL : FWD_L2;
-    hipEvent_t ev[5];
-    if (ms_out) {
-        for (int k = 0; k < 5; ++k) GR_HIP(hipEventCreate(&ev[k]));
-        for (int k = 0; k < 4; ++k) ms_out[k] = 0.0;
-    }
+    double scratch[5] = {0, 0, 0, 0, 0};
+    double* ms = ms_out ? ms_out : scratch;
+    for (int k = 0; k < 5; ++k) ms[k] = 0.0;
+    Timer t(ms_out != nullptr, st);
+    auto fwd_mode = [&](int it) {
+        return pmode | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
+    };
+    const bool fused = s && s->u2 && s->v2 && (!P.tgv || s->w2) && !P.fuse_primal &&
+                       n_iter >= 2;
     int rc = GR_OK;
-    for (int it = 0; it < n_iter && rc == GR_OK; ++it) {
-        if (ms_out) (void)hipEventRecord(ev[0], st);
-        rc = launch_dual(P, st);
-        if (ms_out) (void)hipEventRecord(ev[1], st);
-        const int mode = pmode | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
-        if (!rc) rc = launch_fwd(P, gg->g, nsl, mode, st);
-        if (ms_out) (void)hipEventRecord(ev[2], st);
-        if (!rc) rc = launch_adj(P, nsl, P.fuse_primal != 0, st);
-        if (ms_out) (void)hipEventRecord(ev[3], st);
-        if (!rc && !P.fuse_primal) rc = launch_primal(P, st);
-        if (ms_out) {
-            (void)hipEventRecord(ev[4], st);
-            hipError_t e = hipEventSynchronize(ev[4]);
-            if (e != hipSuccess && !rc) rc = hip_status(e, "tgv_profile");
-            for (int k = 0; k < 4 && !rc; ++k) {
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-                ms_out[k] += ms;
-            }
-        }
+    int it = 0;
+    // unfused iterations: all of them, or one to make the fused count even
+    const int plain = fused ? ((n_iter - 1) % 2) : n_iter;
+    for (; it < plain && rc == GR_OK; ++it) {
+        rc = t.run(&ms[0], [&] { return launch_dual(P, st); });
+        if (!rc) rc = t.run(&ms[1], [&] { return launch_fwd(P, gg->g, nsl, fwd_mode(it), st); });
+        if (!rc) rc = t.run(&ms[2], [&] { return launch_adj(P, nsl, P.fuse_primal != 0, st); });
+        if (!rc && !P.fuse_primal) rc = t.run(&ms[3], [&] { return launch_primal(P, st); });
     }
-    if (ms_out)
-        for (int k = 0; k < 5; ++k) (void)hipEventDestroy(ev[k]);
+    if (rc || it >= n_iter) return rc;
+    // fused form over the remaining n_iter - it iterations
+    FusedBufs A{P.u, P.p, P.v, P.w, s->u2, P.pbar, s->v2, s->w2};
+    FusedBufs Bk{s->u2, P.pbar, s->v2, s->w2, P.u, P.p, P.v, P.w};
+    rc = t.run(&ms[0], [&] { return launch_dual(P, st); });
+    if (!rc) rc = t.run(&ms[1], [&] { return launch_fwd(P, gg->g, nsl, fwd_mode(it), st); });
+    if (!rc) rc = t.run(&ms[2], [&] { return launch_adj(P, nsl, false, st); });
+    for (int k = 0; it + 1 < n_iter && rc == GR_OK; ++k) {
+        const FusedBufs& B = (k % 2 == 0) ? A : Bk;
+        rc = t.run(&ms[4], [&] { return launch_fused_pd(P, B, st); });
+        ++it;
+        if (!rc) rc = t.run(&ms[1], [&] { return launch_fwd(P, gg->g, nsl, fwd_mode(it), st); });
+        if (!rc) rc = t.run(&ms[2], [&] { return launch_adj(P, nsl, false, st); });
+    }
+    if (!rc) rc = t.run(&ms[3], [&] { return launch_primal(P, st); });
     return rc;
 }
 
@@ -571,7 +622,7 @@ int gr_tgv_iterate(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_s
     }
     if (n_iter == 0) return GR_OK;
     const ProbDev P = make_prob(gg, p, s);
-    return iterate_single(gg, P, n_iter, ku_on_last, stream_of(stream), nullptr);
+    return iterate_single(gg, P, n_iter, ku_on_last, stream_of(stream), nullptr, s);
 }
 
 int gr_tgv_profile(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
@@ -582,7 +633,7 @@ int gr_tgv_profile(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_s
         return GR_ERR_INVALID;
     }
     const ProbDev P = make_prob(gg, p, s);
-    return iterate_single(gg, P, n_iter, 0, stream_of(stream), ms_out);
+    return iterate_single(gg, P, n_iter, 0, stream_of(stream), ms_out, s);
 }
 
 int gr_tgv_energy(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,

@@ -65,6 +65,14 @@ struct ProbDev {
     const float *f, *mfac;
 };
 
+// Buffers of the fused primal + next-dual step: the previous state is read
+// from *_in, the next written to *_out (u, p, v, w alternate between the
+// caller's arrays and a second set).
+struct FusedBufs {
+    const float *u_in, *p_in, *v_in, *w_in;
+    float *u_out, *p_out, *v_out, *w_out;
+};
+
 struct FwdItem { int prob, cls, a0, na, b0, nb, s0, pad; };
 struct AdjItem { int prob, x0, y0, s0; };
 // Stencil items: a 64 x 4 tile of plane z (dual step) or of slice
@@ -112,6 +120,7 @@ int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipSt
 // the primal step of the iteration (primal == true)
 int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st);
 int launch_dual(const ProbDev& P, hipStream_t st);
+int launch_fused_pd(const ProbDev& P, const FusedBufs& B, hipStream_t st);
 int launch_primal(const ProbDev& P, hipStream_t st);
 
 // ---- launchers (batches: descriptor table + item table) -------------------

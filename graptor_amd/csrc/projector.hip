@@ -12,6 +12,7 @@
 // backprojection interpolates the sinogram linearly at t.  The two are exact
 // transposes of each other.
 #include "common.h"
+#include "stencil.h"
 
 #include <algorithm>
 #include <cmath>
@@ -236,64 +237,16 @@ __device__ __forceinline__ void primal_update_r(
     int ny, int nz, size_t cs, float hinv, float tau, float norming, size_t i, int x, int y, int z,
     float ktl)
 {
-    const size_t plane = (size_t)nx * ny;
-    float vc[3];
-#pragma unroll
-    for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
-    float d0 = (x == nx - 1) ? 0.f : vc[0];
-    if (x > 0) d0 -= v[i - 1];
-    float d1 = (y == ny - 1) ? 0.f : vc[1];
-    if (y > 0) d1 -= v[i + cs - nx];
-    float dv = d0 + d1;
-    if constexpr (DIM == 3) {
-        float d2 = (z == nz - 1) ? 0.f : vc[2];
-        if (z > 0) d2 -= v[i + 2 * cs - plane];
-        dv += d2 * hinv;
-    }
-    float q[3] = {0.f, 0.f, 0.f};
-    float po[3] = {0.f, 0.f, 0.f};
-    if constexpr (TGV) {
-        constexpr int M = DIM == 2 ? 3 : 6;
-        float wc[M];
-#pragma unroll
-        for (int m = 0; m < M; ++m) wc[m] = w[i + m * cs];
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) po[k] = p[i + k * cs];
-        if constexpr (DIM == 2) {
-            // x-part uses (w_xx, w_xy), y-part (w_xy, w_yy)
-            float x0 = (x == 0) ? 0.f : -wc[0], x1 = (x == 0) ? 0.f : -wc[2];
-            if (x < nx - 1) { x0 += w[i + 1]; x1 += w[i + 2 * cs + 1]; }
-            float y0 = (y == 0) ? 0.f : -wc[2], y1 = (y == 0) ? 0.f : -wc[1];
-            if (y < ny - 1) { y0 += w[i + 2 * cs + nx]; y1 += w[i + cs + nx]; }
-            q[0] = x0 + y0;
-            q[1] = x1 + y1;
-        } else {
-            // tensor order (xx, yy, zz, xy, xz, yz): x-part (0,3,4), y-part (3,1,5),
-            // z-part (4,5,2)
-            const int xi[3] = {0, 3, 4}, yi[3] = {3, 1, 5}, zi[3] = {4, 5, 2};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                float xs = (x == 0) ? 0.f : -wc[xi[k]];
-                if (x < nx - 1) xs += w[i + xi[k] * cs + 1];
-                float ys = (y == 0) ? 0.f : -wc[yi[k]];
-                if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
-                float zs = (z == 0) ? 0.f : -wc[zi[k]];
-                if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
-                q[k] = xs + ys + zs * hinv;
-            }
-        }
-    }
-    const float uo = u[i];
-    float un = uo + tau * (dv - norming * ktl);
-    un = un < 0.f ? 0.f : un;
+    float un, ub, pn[3], pb[3];
+    primal_point<DIM, TGV, true, true>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x,
+                                       y, z, un, ub, pn, pb);
     u[i] = un;
-    ubar[i] = 2.f * un - uo;
+    ubar[i] = ub;
     if constexpr (TGV) {
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            const float pn = po[k] + tau * (vc[k] + q[k]);
-            p[i + k * cs] = pn;
-            pbar[i + k * cs] = 2.f * pn - po[k];
+            p[i + k * cs] = pn[k];
+            pbar[i + k * cs] = pb[k];
         }
     }
 }

@@ -8,6 +8,7 @@
 //                fused with the backprojection.
 // energy kernels: FP64 block partial sums of the primal energy.
 #include "common.h"
+#include "stencil.h"
 
 #include <cmath>
 #include <vector>
@@ -100,20 +101,89 @@ __device__ __forceinline__ void jacobi3(const T (&A)[6], T (&ev)[3], T (&V)[3][3
 
 // ---- dual step ------------------------------------------------------------
 
-template <int DIM, int CPL, bool TGV, int MAXC>
-__device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
+// Sources of the extrapolated iterates the dual step reads.  MemSrc reads
+// the stored ubar / pbar; FusedSrc evaluates the primal step of the
+// previous iteration on the fly (own voxel: stored as the new u, ubar, p;
+// neighbours: recomputed), so that one kernel performs the primal step of
+// iteration k and the dual step of iteration k + 1.
+template <int DIM, bool TGV>
+struct MemSrc {
+    const float* __restrict__ ubar;
+    const float* __restrict__ pbar;
+    size_t cs;
+    __device__ __forceinline__ void own(size_t j, int, int, int, float& ub, float (&pb)[3]) const
+    {
+        ub = ubar[j];
+        if constexpr (TGV) {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) pb[k] = pbar[j + k * cs];
+        }
+    }
+    __device__ __forceinline__ float ubn(size_t j, int, int, int) const { return ubar[j]; }
+    __device__ __forceinline__ void pbn(size_t j, int, int, int, float (&pb)[3]) const
+    {
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) pb[k] = pbar[j + k * cs];
+    }
+};
+
+template <int DIM, bool TGV>
+struct FusedSrc {
+    // previous iterate (read only in the kernel)
+    const float* __restrict__ u;
+    const float* __restrict__ p;
+    const float* __restrict__ v;
+    const float* __restrict__ w;
+    const float* __restrict__ ktl;
+    // primal results of the own voxel
+    float* __restrict__ u_out;
+    float* __restrict__ ubar_out;
+    float* __restrict__ p_out;
+    int nx, ny, nz;
+    size_t cs;
+    float hinv, tau, norming;
+    __device__ __forceinline__ void own(size_t j, int x, int y, int z, float& ub,
+                                        float (&pb)[3]) const
+    {
+        float un, pn[3];
+        primal_point<DIM, TGV, true, true>(u, p, v, w, ktl[j], nx, ny, nz, cs, hinv, tau, norming,
+                                           j, x, y, z, un, ub, pn, pb);
+        u_out[j] = un;
+        ubar_out[j] = ub;
+        if constexpr (TGV) {
+#pragma unroll
+            for (int k = 0; k < DIM; ++k) p_out[j + k * cs] = pn[k];
+        }
+    }
+    __device__ __forceinline__ float ubn(size_t j, int x, int y, int z) const
+    {
+        float un, ub, pn[3], pb[3];
+        primal_point<DIM, TGV, true, false>(u, p, v, w, ktl[j], nx, ny, nz, cs, hinv, tau,
+                                            norming, j, x, y, z, un, ub, pn, pb);
+        return ub;
+    }
+    __device__ __forceinline__ void pbn(size_t j, int x, int y, int z, float (&pb)[3]) const
+    {
+        float un, ub, pn[3];
+        primal_point<DIM, TGV, false, true>(u, p, v, w, 0.f, nx, ny, nz, cs, hinv, tau, norming,
+                                            j, x, y, z, un, ub, pn, pb);
+    }
+};
+
+// Dual step at one voxel, all channels (update_v/update_w + the coupled
+// projections): v = proj(v_in + sigma (grad ubar - pbar)), w = proj(w_in +
+// sigma E pbar), written to v_out / w_out (the same arrays as v / w for the
+// standalone step: each thread reads its own entries before writing them,
+// so they are not declared restrict).
+template <int DIM, int CPL, bool TGV, int MAXC, class Src>
+__device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z, const Src& src,
+                                          const float* v, const float* w, float* vout,
+                                          float* wout)
 {
     constexpr int D = DIM;
     constexpr int M = DIM == 2 ? 3 : 6;
     const int nx = P.nx, ny = P.ny, nz = P.nz, nch = P.nch;
     const float hinv = P.hinv, sigma = P.sigma, ainv0 = P.ainv0, ainv1 = P.ainv1;
-    float* __restrict__ v = P.v;
-    float* __restrict__ w = P.w;
-    const float* __restrict__ ubar = P.ubar;
-    const float* __restrict__ pbar = P.pbar;
-    const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
-    const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
-    const int z = I.z;
     if (x >= nx || y >= ny) return;
     const size_t plane = (size_t)nx * ny;
     const size_t chs = plane * nz;        // channel stride
@@ -127,29 +197,38 @@ __device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
     for (int c = 0; c < MAXC; ++c) {
         if (c >= nch) break;
         const size_t ic = i + c * chs;
-        const float u0 = ubar[ic];
-        float g[3];
-        g[0] = (x < nx - 1) ? ubar[ic + 1] - u0 : 0.f;
-        g[1] = (y < ny - 1) ? ubar[ic + nx] - u0 : 0.f;
-        if constexpr (DIM == 3) g[2] = (z < nz - 1) ? (ubar[ic + plane] - u0) * hinv : 0.f;
+        float u0;
         float p[3] = {0.f, 0.f, 0.f};
-        if constexpr (TGV) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) p[k] = pbar[ic + k * cs];
-        }
+        src.own(ic, x, y, z, u0, p);
+        float g[3];
+        g[0] = (x < nx - 1) ? src.ubn(ic + 1, x + 1, y, z) - u0 : 0.f;
+        g[1] = (y < ny - 1) ? src.ubn(ic + nx, x, y + 1, z) - u0 : 0.f;
+        if constexpr (DIM == 3)
+            g[2] = (z < nz - 1) ? (src.ubn(ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             vv[c][k] = v[ic + k * cs] + sigma * (g[k] - p[k]);
             nv2 += vv[c][k] * vv[c][k];
         }
         if constexpr (TGV) {
-            float ax[3], ay[3], az[3] = {0.f, 0.f, 0.f};
+            float ax[3] = {0.f, 0.f, 0.f}, ay[3] = {0.f, 0.f, 0.f}, az[3] = {0.f, 0.f, 0.f};
+            float q[3];
+            if (x > 0) {
+                src.pbn(ic - 1, x - 1, y, z, q);
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                ax[k] = (x > 0) ? p[k] - pbar[ic - 1 + k * cs] : 0.f;
-                ay[k] = (y > 0) ? p[k] - pbar[ic - nx + k * cs] : 0.f;
-                if constexpr (DIM == 3)
-                    az[k] = (z > 0) ? (p[k] - pbar[ic - plane + k * cs]) * hinv : 0.f;
+                for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
+            }
+            if (y > 0) {
+                src.pbn(ic - nx, x, y - 1, z, q);
+#pragma unroll
+                for (int k = 0; k < D; ++k) ay[k] = p[k] - q[k];
+            }
+            if constexpr (DIM == 3) {
+                if (z > 0) {
+                    src.pbn(ic - plane, x, y, z - 1, q);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) az[k] = (p[k] - q[k]) * hinv;
+                }
             }
             float e[6];
             e[0] = ax[0];
@@ -267,12 +346,22 @@ __device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
         if (c >= nch) break;
         const size_t ic = i + c * chs;
 #pragma unroll
-        for (int k = 0; k < D; ++k) v[ic + k * cs] = vv[c][k];
+        for (int k = 0; k < D; ++k) vout[ic + k * cs] = vv[c][k];
         if constexpr (TGV) {
 #pragma unroll
-            for (int m = 0; m < M; ++m) w[ic + m * cs] = ww[c][m] * wsc;
+            for (int m = 0; m < M; ++m) wout[ic + m * cs] = ww[c][m] * wsc;
         }
     }
+}
+
+template <int DIM, int CPL, bool TGV, int MAXC>
+__device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
+{
+    const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
+    const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
+    const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;
+    const MemSrc<DIM, TGV> src{P.ubar, P.pbar, cs};
+    dual_core<DIM, CPL, TGV, MAXC>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
 }
 
 // Work order: x tiles fastest, then rows, then planes (plane-major order
@@ -446,6 +535,78 @@ static void dual_single_launch(const ProbDev& P, hipStream_t st)
     const long long blocks = (long long)tiles_x * tiles_y * P.nz;
     hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC>), dim3((unsigned)blocks),
                        dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+}
+
+// Primal step of iteration k fused with the dual step of iteration k + 1:
+// reads the previous iterate (u, p, v, w) from the *_in buffers and K^T lam
+// from P.ktl, writes u, ubar (P.ubar), p, v, w of the next state to the
+// *_out buffers.  Inputs and outputs are distinct arrays, so neighbours can
+// be evaluated from unmodified inputs.
+template <int DIM, int CPL, bool TGV, int MAXC>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void pd_fused_kernel(ProbDev P, FusedBufs B,
+                                                                    int tiles_x, int tiles_y)
+{
+    const int b = blockIdx.x;
+    const int tx = b % tiles_x, rest = b / tiles_x;
+    const int x = tx * DUAL_BX + (int)(threadIdx.x & (DUAL_BX - 1));
+    const int y = (rest % tiles_y) * DUAL_BY + (int)(threadIdx.x / DUAL_BX);
+    const int z = rest / tiles_y;
+    const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;
+    const FusedSrc<DIM, TGV> src{B.u_in, B.p_in, B.v_in, B.w_in, P.ktl, B.u_out, P.ubar,
+                                 B.p_out, P.nx, P.ny, P.nz, cs, P.hinv, P.tau, P.norming};
+    dual_core<DIM, CPL, TGV, MAXC>(P, x, y, z, src, B.v_in, B.w_in, B.v_out, B.w_out);
+}
+
+template <int DIM, int CPL, bool TGV, int MAXC>
+static void fused_launch(const ProbDev& P, const FusedBufs& B, hipStream_t st)
+{
+    const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
+    const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
+    hipLaunchKernelGGL((pd_fused_kernel<DIM, CPL, TGV, MAXC>),
+                       dim3((unsigned)(tiles_x * tiles_y * P.nz)), dim3(DUAL_BX * DUAL_BY), 0, st,
+                       P, B, tiles_x, tiles_y);
+}
+
+template <int DIM, int CPL, bool TGV>
+static void fused_c(const ProbDev& P, const FusedBufs& B, hipStream_t st)
+{
+    if (P.nch <= 1) fused_launch<DIM, CPL, TGV, 1>(P, B, st);
+    else if (P.nch <= 2) fused_launch<DIM, CPL, TGV, 2>(P, B, st);
+    else if (P.nch <= 4) fused_launch<DIM, CPL, TGV, 4>(P, B, st);
+    else fused_launch<DIM, CPL, TGV, 8>(P, B, st);
+}
+
+template <int DIM, int CPL>
+static void fused_t(const ProbDev& P, const FusedBufs& B, hipStream_t st)
+{
+    if (P.tgv) fused_c<DIM, CPL, true>(P, B, st);
+    else fused_c<DIM, CPL, false>(P, B, st);
+}
+
+template <int DIM>
+static void fused_d(const ProbDev& P, const FusedBufs& B, hipStream_t st)
+{
+    if (P.coupling == GR_COUPLING_UNCORRELATED) fused_t<DIM, GR_COUPLING_UNCORRELATED>(P, B, st);
+    else if (P.coupling == GR_COUPLING_FROBENIUS) fused_t<DIM, GR_COUPLING_FROBENIUS>(P, B, st);
+    else fused_t<DIM, GR_COUPLING_NUCLEAR>(P, B, st);
+}
+
+int launch_fused_pd(const ProbDev& P, const FusedBufs& B, hipStream_t st)
+{
+    if (P.nch > 8) {
+        set_error("fused step: %d channels exceed the 8 this build holds per voxel", P.nch);
+        return GR_ERR_UNSUPPORTED;
+    }
+    const long long blocks = (long long)((P.nx + DUAL_BX - 1) / DUAL_BX) *
+                             ((P.ny + DUAL_BY - 1) / DUAL_BY) * P.nz;
+    if (blocks > 0x7fffffff) {
+        set_error("fused step: grid too large");
+        return GR_ERR_INVALID;
+    }
+    if (P.dim == 2) fused_d<2>(P, B, st);
+    else fused_d<3>(P, B, st);
+    GR_LAUNCH_CHECK("pd_fused_kernel");
+    return GR_OK;
 }
 
 template <int DIM, int CPL, bool TGV>

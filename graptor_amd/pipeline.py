@@ -197,8 +197,16 @@ def write_results(result, channelnames, solution, sinogram):
         gio.write_mrc(f"{result}_all_channels_normalized.mrc", norm)
 
 
-def _prepare(names, section, datahandling, scalepars, dim):
+def _prepare(names, section, datahandling, scalepars, dim, reductionparameter=(0, None)):
     brightness, baseline, cutedges, badfile, threshold, bad = datahandling
+    bad = list(bad)
+    fraction, radius = reductionparameter
+    if fraction != 0:
+        # RC:319-320 appends the detected list as one element of the list,
+        # which the membership test of H5:70 then never matches, so the
+        # reference drops nothing; the detected projections are removed here
+        from .badprojections import find_bad_projections
+        bad.extend(find_bad_projections(names[0], radius, fraction))
     coll, ratio, noise, angles = [], [], [], None
     for name in names:
         data, angles, avg = load_preprocessed(name, brightness, badfile, float(cutedges), bad)
@@ -214,17 +222,18 @@ def _prepare(names, section, datahandling, scalepars, dim):
 def coupled_reconstruction_3d(names, result, mu, alpha, maxiter, method, sinogram_level,
                               datahandling, channelnames, regularisation="TGV",
                               discrepancy="KL", overlay=1, chunksize=0, scalepars=(1, 98),
-                              device=None, write=True, L=None):
+                              device=None, write=True, L=None, reductionparameter=(0, None)):
     """Counterpart of Coupled_Reconstruction3d (RC:294-540).  sinogram_level
     = (first, last, step); datahandling = [brightness, baseline, cutedges,
-    badfile, threshold, bad_projections].  Returns U (C, Nx, Nx, nslices)
+    badfile, threshold, bad_projections]; reductionparameter = (fraction or
+    count, search radius) of the bad-projection detection (RC:319-320).  Returns U (C, Nx, Nx, nslices)
     float64 and the rescaled (Solution, Sinogram) arrays."""
     from . import radon_tgv_primal_dual_3d as r3d
     from .solver import resolve_device
 
     section = list(range(sinogram_level[0], sinogram_level[1] + 1, sinogram_level[2]))
     coll, ratio, noise, angles, baseline, threshold = _prepare(names, section, datahandling,
-                                                               scalepars, 3)
+                                                               scalepars, 3, reductionparameter)
     sinos = [apply_baseline(d, baseline, threshold, noise[i], 3)[:, section, :]
              for i, d in enumerate(coll)]
     C = len(sinos)
@@ -270,7 +279,7 @@ def coupled_reconstruction_3d(names, result, mu, alpha, maxiter, method, sinogra
 def coupled_reconstruction_2d(names, result, mu, alpha, maxiter, method, slice_levels,
                               datahandling, channelnames, regularisation="TGV",
                               discrepancy="KL", scalepars=(1, 98), device=None, write=True,
-                              L=None, batch=True):
+                              L=None, batch=True, reductionparameter=(0, None)):
     """Counterpart of Coupled_Reconstruction2d (RC:109-292): every selected
     slice is an independent 2D problem.  With ``batch`` they are solved
     together in one batch plan (one launch per kernel class per phase);
@@ -286,7 +295,8 @@ def coupled_reconstruction_2d(names, result, mu, alpha, maxiter, method, slice_l
 
     considered = list(range(slice_levels[0], slice_levels[1] + 1, slice_levels[2]))
     coll, ratio, noise, angles, baseline, threshold = _prepare(names, considered,
-                                                               datahandling, scalepars, 2)
+                                                               datahandling, scalepars, 2,
+                                                               reductionparameter)
     C = len(coll)
     na, _, nd = coll[0].shape
     Nz = len(considered)
@@ -317,7 +327,7 @@ def coupled_reconstruction_2d(names, result, mu, alpha, maxiter, method, slice_l
             f = np.stack([np.asarray(s, np.float32) for s in sinos[j]])[:, None]
             probs.append(TgvProblem(f, geom, alpha, mu, dim=2, coupling=coupling,
                                     discrepancy=discrepancy, regularisation=regularisation,
-                                    L=L, device=dev))
+                                    L=L, device=dev, fused=False))
         plan = BatchPlan(probs)
         plan.iterate(int(maxiter), ku_on_last=True)
         torch.cuda.synchronize(dev)

@@ -84,12 +84,20 @@ class TgvProblem:
     dim      : 2 (Nz must be 1) or 3.
     L        : operator norm to use; None estimates it (``normest_mode``).
     u0       : optional warm start (C, Nz, nd, nd); the reference starts at 0.
+    fused    : allocate the second state buffers with which ``iterate`` runs
+               the primal step and the next dual step as one kernel (same
+               values; 112 instead of 176 algorithmic bytes per voxel-channel,
+               ~10 more floats of memory per voxel-channel).  Off by default:
+               the fused kernel re-evaluates the primal step at the six
+               neighbours it needs and measured 1.47 ms against 0.70 ms for
+               the two separate kernels on 256x256x64x4.
     """
 
     def __init__(self, f, geometry: RadonGeometry, alpha, mu, dim=3,
                  coupling="frobenius", discrepancy="KL", regularisation="TGV",
                  stepsize=1.0, L=None, normest_mode="reference", normest_seed=None,
-                 u0=None, feasible=True, device=None):
+                 u0=None, feasible=True, device=None, fused=False):
+        self.fused = bool(fused)
         if coupling not in _lib.COUPLING:
             raise ValueError(f"unknown coupling {coupling!r}")
         if discrepancy not in _lib.DISCREPANCY:
@@ -151,6 +159,11 @@ class TgvProblem:
         # K^T lam: needed unless the geometry fuses the primal step into the
         # backprojection
         self.ktl = None if getattr(self.geometry, "fuse_primal", False) else z(*img)
+        # second state buffers of the fused primal + next-dual kernel
+        alt = self.fused and self.ktl is not None
+        self.u2 = z(*img) if alt else None
+        self.v2 = z(d, *img) if alt else None
+        self.w2 = z(m, *img) if alt and tgv else None
         self.ku = torch.zeros_like(self.f)
         self.mfac = torch.as_tensor(self.k.mfac.astype(np.float32), device=self.device)
         self.params = _lib.TgvParams(
@@ -165,6 +178,7 @@ class TgvProblem:
         self.state = _lib.TgvState(
             u=ptr(self.u), ubar=ptr(self.ubar), p=ptr(self.p), pbar=ptr(self.pbar),
             v=ptr(self.v), w=ptr(self.w), lam=ptr(self.lam), ktl=ptr(self.ktl),
+            u2=ptr(self.u2), v2=ptr(self.v2), w2=ptr(self.w2),
             ku=ptr(self.ku), f=ptr(self.f), mfac=ptr(self.mfac))
         self.iterations = 0
 
@@ -183,9 +197,10 @@ class TgvProblem:
 
     def profile(self, n: int):
         """Run ``n`` iterations with HIP events around each kernel; returns the
-        summed milliseconds of (dual step, forward projection + prox,
-        backprojection [+ primal step when fused], primal step)."""
-        out = (ctypes.c_double * 4)()
+        summed milliseconds of (standalone dual steps, forward projections +
+        prox, backprojections [+ primal step when the geometry fuses it],
+        standalone primal steps, fused primal + next-dual steps)."""
+        out = (ctypes.c_double * 5)()
         with torch.cuda.device(self.device):
             _lib.check(_lib.load().gr_tgv_profile(
                 self.geometry.handle, ctypes.byref(self.params), ctypes.byref(self.state),
@@ -219,12 +234,16 @@ class TgvProblem:
         for name in ("f", "u", "ubar", "p", "pbar", "v", "w", "lam", "ktl", "ku", "mfac"):
             t = getattr(self, name)
             setattr(other, name, None if t is None else t.clone())
+        for name in ("u2", "v2", "w2"):  # scratch: contents never carried over
+            t = getattr(self, name, None)
+            setattr(other, name, None if t is None else torch.zeros_like(t))
         other.params = _lib.TgvParams.from_buffer_copy(self.params)
         ptr = lambda t: t.data_ptr() if t is not None else None
         other.state = _lib.TgvState(
             u=ptr(other.u), ubar=ptr(other.ubar), p=ptr(other.p), pbar=ptr(other.pbar),
             v=ptr(other.v), w=ptr(other.w), lam=ptr(other.lam), ktl=ptr(other.ktl),
-            ku=ptr(other.ku), f=ptr(other.f), mfac=ptr(other.mfac))
+            ku=ptr(other.ku), f=ptr(other.f), mfac=ptr(other.mfac),
+            u2=ptr(other.u2), v2=ptr(other.v2), w2=ptr(other.w2))
         return other
 
 

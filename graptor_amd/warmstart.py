@@ -422,7 +422,7 @@ class SliceBatch:
                        coupling=cfg.coupling if cfg.C > 1 else "uncorrelated",
                        discrepancy=cfg.discrepancy, L=L, feasible=False,
                        u0=None if u0 is None else u0[b].view(cfg.C, 1, cfg.N, cfg.N),
-                       device=f.device)
+                       device=f.device, fused=False)
             for b in range(B)]
         self.plan = BatchPlan(self.problems)
 

@@ -105,11 +105,17 @@ typedef struct gr_tgv_params {
 /* Caller-owned device state.  u, ubar: nch*nz*ny*nx; p, pbar, v: dim
  * components; w: 3 (2D) or 6 (3D) components; lam, f, ktl: sinogram /
  * image stacks; mfac: nch floats (mu for KL, mu/(sigma+mu) for L2); ku may
- * be NULL unless ku_on_last or gr_tgv_energy needs it. */
+ * be NULL unless ku_on_last or gr_tgv_energy needs it.  u2, v2, w2 (shaped
+ * as u, v, w) are optional second buffers: when given (w2 may be NULL for
+ * TV), gr_tgv_iterate runs the primal step of each iteration and the dual
+ * step of the next as one kernel, alternating the state between the two
+ * sets (pbar serves as the second p); the state ends in u, p, v, w as
+ * without them and the values are the same. */
 typedef struct gr_tgv_state {
     float *u, *ubar, *p, *pbar, *v, *w;
     float *lam, *ktl, *ku;
     const float *f, *mfac;
+    float *u2, *v2, *w2;
 } gr_tgv_state;
 
 /* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 on `stream`:
@@ -124,10 +130,11 @@ int gr_tgv_iterate(const gr_geometry *g, const gr_tgv_params *prm,
                    void *stream);
 
 /* gr_tgv_iterate with a HIP event pair around each kernel of every
- * iteration (measurement only; no counterpart in the reference).  ms_out[0..3]
- * receive the summed kernel times in milliseconds of the dual step, the
- * forward projection + prox, the backprojection (+ primal step when fused)
- * and the primal step (0 when fused).  Synchronises the stream. */
+ * iteration (measurement only; no counterpart in the reference).  ms_out[0..4]
+ * receive the summed kernel times in milliseconds of the standalone dual
+ * steps, the forward projections + prox, the backprojections (+ primal step
+ * when the geometry fuses it), the standalone primal steps and the fused
+ * primal + next-dual steps.  Synchronises the stream. */
 int gr_tgv_profile(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, double *ms_out, void *stream);
 
