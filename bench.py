@@ -214,6 +214,66 @@ def warmstart_main(args):
     print(json.dumps(line, default=float), flush=True)
 
 
+def unet_main(args):
+    """The warm-start network's kernels alone: Adam steps of the 4-level
+    U-Net (base 32) on a batch of 16 random 256 x 256 x 4 inputs and forward
+    passes of a batch of 32, timed with HIP events; MFMA roofline of the
+    training step (3 x forward flops) and of inference."""
+    import torch
+
+    from graptor_amd.warmstart import UNet, conv_flops
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    net = UNet(cin=4, cout=4, base=32, device=dev, seed=0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    X = torch.randn((16, 256, 256, 4), device=dev, generator=g)
+    T = torch.randn((16, 256, 256, 4), device=dev, generator=g)
+    Xi = torch.randn((32, 256, 256, 4), device=dev, generator=g)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+
+    def train_step():
+        net.forward(X)
+        net.backward(T, return_loss=False)
+        net.adam()
+
+    for _ in range(max(args.warmup, 1)):
+        train_step()
+        net.forward(Xi)
+    steps = args.steps
+    a, b = ev(), ev()
+    a.record()
+    for _ in range(steps):
+        train_step()
+    b.record()
+    b.synchronize()
+    train_ms = a.elapsed_time(b) / steps
+    a.record()
+    for _ in range(steps):
+        net.forward(Xi)
+    b.record()
+    b.synchronize()
+    infer_ms = a.elapsed_time(b) / steps
+    f = conv_flops(net, 256, 256)
+    train_tf = 3 * f * 16 / (train_ms * 1e-3) / 1e12
+    infer_tf = f * 32 / (infer_ms * 1e-3) / 1e12
+    line = {
+        "metric": "U-Net training / inference throughput (warm-start network)",
+        "value": 16 / (train_ms * 1e-3), "unit": "training images/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": train_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 (fp32 accumulate)", "data": "synthetic (random)",
+        "config": {"workload": "4-level U-Net, base 32, 1.95 M parameters, batch 16 "
+                               "(training) / 32 (inference), 256x256x4"},
+        "roofline": {"bound": "mfma", "achieved": infer_tf, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": infer_tf / 2500.0, "traffic": None,
+                     "kernel": "forward pass (conv_fwd_kernel and the elementwise kernels)"},
+        "train_tflops": train_tf, "inference_ms_batch32": infer_ms,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,7 +284,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--workload", default="coupled3d",
-                    choices=["coupled3d", "large3d", "batched2d", "warmstart"])
+                    choices=["coupled3d", "large3d", "batched2d", "warmstart", "unet"])
     ap.add_argument("--cap-iters", type=int, default=2000,
                     help="large3d: iteration cap of the run to a relative-change tolerance")
     ap.add_argument("--change-tol", type=float, default=1e-3)
@@ -252,6 +312,8 @@ def main():
         return batched_main(args)
     if args.workload == "warmstart":
         return warmstart_main(args)
+    if args.workload == "unet":
+        return unet_main(args)
 
     import numpy as np
     import torch

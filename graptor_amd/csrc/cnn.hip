@@ -203,14 +203,24 @@ __global__ __launch_bounds__(256) void conv_wgrad_kernel(
     }
 }
 
-// db[co] += sum over pixels of dy[p][co] (FP32 accumulation, atomics per block)
-__global__ void bias_grad_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0, size_t npix,
-                                 int cout, float* __restrict__ db)
+// db[co] += sum over pixels of dy[p][co] (FP32 accumulation).  A block of
+// 256 threads covers 256 / cout pixel lanes x cout channels (cout divides
+// 256); lanes are summed in LDS and each block adds one partial per channel.
+__global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restrict__ dy, int ldd,
+                                                        int cd0, size_t npix, int cout,
+                                                        float* __restrict__ db)
 {
-    const int co = threadIdx.x;  // blockDim.x == cout (<= 1024)
+    __shared__ float red[256];
+    const int co = threadIdx.x % cout, lanes = 256 / cout, lane = threadIdx.x / cout;
     float acc = 0.f;
-    for (size_t p = blockIdx.x; p < npix; p += gridDim.x) acc += bf2f(dy[p * ldd + cd0 + co]);
-    atomicAdd(&db[co], acc);
+    for (size_t p = (size_t)blockIdx.x * lanes + lane; p < npix; p += (size_t)gridDim.x * lanes)
+        acc += bf2f(dy[p * ldd + cd0 + co]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (lane == 0) {
+        for (int l = 1; l < lanes; ++l) acc += red[l * cout + co];
+        atomicAdd(&db[co], acc);
+    }
 }
 
 // Wt[ci][8 - tap][co] = W[co][tap][ci]: weights of the data-gradient
@@ -314,6 +324,122 @@ __global__ void up_bwd_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0,
     dx[((size_t)(im * H + yc) * W + xc) * ldx + cx0 + ch] = f2bf(s);
 }
 
+// ---- 8-channel (16-byte) forms of the elementwise kernels, used when every
+// channel count, stride and offset is a multiple of 8 ----------------------
+__device__ __forceinline__ U16x8 ld8(const uint16_t* p) { return *reinterpret_cast<const U16x8*>(p); }
+__device__ __forceinline__ void st8(uint16_t* p, const U16x8& v) { *reinterpret_cast<U16x8*>(p) = v; }
+
+__global__ void relu_bwd8_kernel(uint16_t* __restrict__ dy, int ldd, int cd0,
+                                 const uint16_t* __restrict__ y, int ldy, int cy0, size_t npix,
+                                 int c8)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix * c8) return;
+    const size_t p = i / c8;
+    const int ch = (int)(i % c8) * 8;
+    uint16_t* d = dy + p * ldd + cd0 + ch;
+    U16x8 g = ld8(d);
+    const U16x8 v = ld8(y + p * ldy + cy0 + ch);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (!(bf2f(v.v[j]) > 0.f)) g.v[j] = 0;
+    st8(d, g);
+}
+
+__global__ void pool_fwd8_kernel(const uint16_t* __restrict__ x, int ldx, int cx0,
+                                 uint16_t* __restrict__ y, int ldy, int cy0, int n, int H, int W,
+                                 int c8)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * Ho * Wo * c8) return;
+    const int ch = (int)(i % c8) * 8;
+    size_t q = i / c8;
+    const int xo = (int)(q % Wo); q /= Wo;
+    const int yo = (int)(q % Ho);
+    const int im = (int)(q / Ho);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+            const U16x8 v =
+                ld8(x + ((size_t)(im * H + 2 * yo + dy) * W + 2 * xo + dx) * ldx + cx0 + ch);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += bf2f(v.v[j]);
+        }
+    U16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.v[j] = f2bf(0.25f * s[j]);
+    st8(y + ((size_t)(im * Ho + yo) * Wo + xo) * ldy + cy0 + ch, o);
+}
+
+__global__ void pool_bwd8_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0,
+                                 uint16_t* __restrict__ dx, int ldx, int cx0, int n, int H, int W,
+                                 int c8, int accumulate)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * H * W * c8) return;
+    const int ch = (int)(i % c8) * 8;
+    size_t q = i / c8;
+    const int xf = (int)(q % W); q /= W;
+    const int yf = (int)(q % H);
+    const int im = (int)(q / H);
+    const int Ho = H / 2, Wo = W / 2;
+    const U16x8 g = ld8(dy + ((size_t)(im * Ho + yf / 2) * Wo + xf / 2) * ldd + cd0 + ch);
+    uint16_t* o = dx + ((size_t)(im * H + yf) * W + xf) * ldx + cx0 + ch;
+    U16x8 r;
+    if (accumulate) {
+        const U16x8 a = ld8(o);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r.v[j] = f2bf(0.25f * bf2f(g.v[j]) + bf2f(a.v[j]));
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r.v[j] = f2bf(0.25f * bf2f(g.v[j]));
+    }
+    st8(o, r);
+}
+
+__global__ void up_fwd8_kernel(const uint16_t* __restrict__ x, int ldx, int cx0,
+                               uint16_t* __restrict__ y, int ldy, int cy0, int n, int H, int W,
+                               int c8)
+{
+    const int Ho = 2 * H, Wo = 2 * W;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * Ho * Wo * c8) return;
+    const int ch = (int)(i % c8) * 8;
+    size_t q = i / c8;
+    const int xo = (int)(q % Wo); q /= Wo;
+    const int yo = (int)(q % Ho);
+    const int im = (int)(q / Ho);
+    st8(y + ((size_t)(im * Ho + yo) * Wo + xo) * ldy + cy0 + ch,
+        ld8(x + ((size_t)(im * H + yo / 2) * W + xo / 2) * ldx + cx0 + ch));
+}
+
+__global__ void up_bwd8_kernel(const uint16_t* __restrict__ dy, int ldd, int cd0,
+                               uint16_t* __restrict__ dx, int ldx, int cx0, int n, int H, int W,
+                               int c8)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * H * W * c8) return;
+    const int ch = (int)(i % c8) * 8;
+    size_t q = i / c8;
+    const int xc = (int)(q % W); q /= W;
+    const int yc = (int)(q % H);
+    const int im = (int)(q / H);
+    const int Wf = 2 * W, Hf = 2 * H;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int a = 0; a < 2; ++a)
+        for (int bb = 0; bb < 2; ++bb) {
+            const U16x8 v =
+                ld8(dy + ((size_t)(im * Hf + 2 * yc + a) * Wf + 2 * xc + bb) * ldd + cd0 + ch);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] += bf2f(v.v[j]);
+        }
+    U16x8 o;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) o.v[j] = f2bf(s[j]);
+    st8(dx + ((size_t)(im * H + yc) * W + xc) * ldx + cx0 + ch, o);
+}
+
 // y = bf16(x) for a dense float array (with output stride / offset per pixel)
 __global__ void pack_bf16_kernel(const float* __restrict__ x, int c_in, uint16_t* __restrict__ y,
                                  int ldy, int cy0, size_t npix, float scale)
@@ -326,12 +452,15 @@ __global__ void pack_bf16_kernel(const float* __restrict__ x, int c_in, uint16_t
 }
 
 // MSE loss gradient: g = (pred - target) * gscale as bf16 into an ld-strided
-// buffer (first c channels); loss partial sums into *loss (atomic)
-__global__ void mse_grad_kernel(const float* __restrict__ pred, int ldp,
-                                const float* __restrict__ target, int c, size_t npix,
-                                uint16_t* __restrict__ g, int ldg, float gscale,
-                                float* __restrict__ loss)
+// buffer (first c channels); when loss is given, the sum of squares is added
+// to *loss (one atomic per block)
+__global__ __launch_bounds__(256) void mse_grad_kernel(const float* __restrict__ pred, int ldp,
+                                                       const float* __restrict__ target, int c,
+                                                       size_t npix, uint16_t* __restrict__ g,
+                                                       int ldg, float gscale,
+                                                       float* __restrict__ loss)
 {
+    __shared__ float red[4];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     float d = 0.f;
     if (i < npix * c) {
@@ -340,9 +469,12 @@ __global__ void mse_grad_kernel(const float* __restrict__ pred, int ldp,
         d = pred[p * ldp + ch] - target[i];
         g[p * ldg + ch] = f2bf(d * gscale);
     }
+    if (!loss) return;  // uniform over the grid
     float s = d * d;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
 }
 
 // Adam on FP32 master weights; also refreshes the bf16 copy
@@ -376,6 +508,11 @@ static unsigned nblk(size_t n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
 // a channel slice [c0, c0 + c) of a buffer with ld channels per pixel
 static bool slice_ok(int ld, int c0, int c) { return c > 0 && c0 >= 0 && c0 + c <= ld; }
+// 16-byte (8-channel) vector access is aligned for this slice
+static bool vec8(int c, int ld, int c0, const void* base)
+{
+    return c % 8 == 0 && ld % 8 == 0 && c0 % 8 == 0 && ((uintptr_t)base & 15) == 0;
+}
 
 extern "C" {
 
@@ -421,12 +558,14 @@ int gr_cnn_conv_wgrad(const void* x, int ldx, int cx0, const void* dy, int ldd, 
 int gr_cnn_bias_grad(const void* dy, int ldd, int cd0, size_t npix, int cout, float* db,
                      void* stream)
 {
-    if (!dy || !db || npix == 0 || cout <= 0 || cout > 1024 || !slice_ok(ldd, cd0, cout)) {
-        set_error("cnn_bias_grad: invalid arguments");
+    if (!dy || !db || npix == 0 || cout <= 0 || cout > 256 || 256 % cout ||
+        !slice_ok(ldd, cd0, cout)) {
+        set_error("cnn_bias_grad: invalid arguments (cout must divide 256)");
         return GR_ERR_INVALID;
     }
-    const unsigned blocks = (unsigned)std::min<size_t>(1024, npix);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks), dim3(cout), 0, cnn_stream(stream),
+    const size_t lanes = 256 / cout;
+    const unsigned blocks = (unsigned)std::min<size_t>(2048, (npix + lanes - 1) / lanes);
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(blocks), dim3(256), 0, cnn_stream(stream),
                        (const uint16_t*)dy, ldd, cd0, npix, cout, db);
     GR_LAUNCH_CHECK("bias_grad_kernel");
     return GR_OK;
@@ -452,9 +591,14 @@ int gr_cnn_relu_bwd(void* dy, int ldd, int cd0, const void* y, int ldy, int cy0,
         set_error("cnn_relu_bwd: invalid arguments");
         return GR_ERR_INVALID;
     }
-    hipLaunchKernelGGL(relu_bwd_kernel, dim3(nblk(npix * c, 256)), dim3(256), 0,
-                       cnn_stream(stream), (uint16_t*)dy, ldd, cd0, (const uint16_t*)y, ldy, cy0,
-                       npix, c);
+    if (vec8(c, ldd, cd0, dy) && vec8(c, ldy, cy0, y))
+        hipLaunchKernelGGL(relu_bwd8_kernel, dim3(nblk(npix * (c / 8), 256)), dim3(256), 0,
+                           cnn_stream(stream), (uint16_t*)dy, ldd, cd0, (const uint16_t*)y, ldy,
+                           cy0, npix, c / 8);
+    else
+        hipLaunchKernelGGL(relu_bwd_kernel, dim3(nblk(npix * c, 256)), dim3(256), 0,
+                           cnn_stream(stream), (uint16_t*)dy, ldd, cd0, (const uint16_t*)y, ldy,
+                           cy0, npix, c);
     GR_LAUNCH_CHECK("relu_bwd_kernel");
     return GR_OK;
 }
@@ -467,11 +611,20 @@ int gr_cnn_pool(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, int 
         set_error("cnn_pool: invalid arguments (even sizes, channel slices within ld)");
         return GR_ERR_INVALID;
     }
-    if (!backward)
+    const bool v8 = vec8(c, ldx, cx0, x) && vec8(c, ldy, cy0, y);
+    if (!backward && v8)
+        hipLaunchKernelGGL(pool_fwd8_kernel, dim3(nblk((size_t)n * H / 2 * W / 2 * (c / 8), 256)),
+                           dim3(256), 0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0,
+                           (uint16_t*)y, ldy, cy0, n, H, W, c / 8);
+    else if (!backward)
         hipLaunchKernelGGL(pool_fwd_kernel, dim3(nblk((size_t)n * H / 2 * W / 2 * c, 256)),
                            dim3(256), 0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0,
                            (uint16_t*)y, ldy, cy0, n, H, W, c);
-    else  // x = coarse gradient, y = fine gradient (H, W fine)
+    else if (v8)  // x = coarse gradient, y = fine gradient (H, W fine)
+        hipLaunchKernelGGL(pool_bwd8_kernel, dim3(nblk((size_t)n * H * W * (c / 8), 256)),
+                           dim3(256), 0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0,
+                           (uint16_t*)y, ldy, cy0, n, H, W, c / 8, accumulate);
+    else
         hipLaunchKernelGGL(pool_bwd_kernel, dim3(nblk((size_t)n * H * W * c, 256)), dim3(256), 0,
                            cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
                            cy0, n, H, W, c, accumulate);
@@ -487,11 +640,20 @@ int gr_cnn_upsample(const void* x, int ldx, int cx0, void* y, int ldy, int cy0, 
         set_error("cnn_upsample: invalid arguments");
         return GR_ERR_INVALID;
     }
-    if (!backward)
+    const bool v8 = vec8(c, ldx, cx0, x) && vec8(c, ldy, cy0, y);
+    if (!backward && v8)
+        hipLaunchKernelGGL(up_fwd8_kernel, dim3(nblk((size_t)n * 4 * H * W * (c / 8), 256)),
+                           dim3(256), 0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0,
+                           (uint16_t*)y, ldy, cy0, n, H, W, c / 8);
+    else if (!backward)
         hipLaunchKernelGGL(up_fwd_kernel, dim3(nblk((size_t)n * 4 * H * W * c, 256)), dim3(256),
                            0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
                            cy0, n, H, W, c);
-    else  // x = fine gradient, y = coarse gradient (H, W coarse)
+    else if (v8)  // x = fine gradient, y = coarse gradient (H, W coarse)
+        hipLaunchKernelGGL(up_bwd8_kernel, dim3(nblk((size_t)n * H * W * (c / 8), 256)),
+                           dim3(256), 0, cnn_stream(stream), (const uint16_t*)x, ldx, cx0,
+                           (uint16_t*)y, ldy, cy0, n, H, W, c / 8);
+    else
         hipLaunchKernelGGL(up_bwd_kernel, dim3(nblk((size_t)n * H * W * c, 256)), dim3(256), 0,
                            cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (uint16_t*)y, ldy,
                            cy0, n, H, W, c);
@@ -515,7 +677,7 @@ int gr_cnn_pack_bf16(const float* x, int c_in, void* y, int ldy, int cy0, size_t
 int gr_cnn_mse_grad(const float* pred, int ldp, const float* target, int c, size_t npix, void* g,
                     int ldg, float gscale, float* loss, void* stream)
 {
-    if (!pred || !target || !g || !loss || npix == 0 || !slice_ok(ldp, 0, c) ||
+    if (!pred || !target || !g || npix == 0 || !slice_ok(ldp, 0, c) ||
         !slice_ok(ldg, 0, c)) {
         set_error("cnn_mse_grad: invalid arguments");
         return GR_ERR_INVALID;

@@ -244,12 +244,15 @@ class UNet:
         n, H, W = self._batch
         npix = n * H * W
         self.g.zero_()
-        self._loss.zero_()
+        if return_loss:
+            self._loss.zero_()
         tgt = target_f32.contiguous()
         total = npix * self.cout
         _lib.check(lib.gr_cnn_mse_grad(self._out.data_ptr(), self.head_out, tgt.data_ptr(),
                                        self.cout, npix, self._dout.data_ptr(), self.head_out,
-                                       2.0 / total, self._loss.data_ptr(), st), "mse")
+                                       2.0 / total,
+                                       self._loss.data_ptr() if return_loss else None, st),
+                   "mse")
         io, gio = self._io()
         b, g = self._bufs, self._grads
         c0, c1, c2, c3 = self.base, 2 * self.base, 4 * self.base, 8 * self.base

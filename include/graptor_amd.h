@@ -195,7 +195,7 @@ int gr_cnn_conv_fwd(const void *x, int ldx, int cx0, const void *w, const float 
 int gr_cnn_conv_wgrad(const void *x, int ldx, int cx0, const void *dy, int ldd, int cd0,
                       float *dw, int n, int H, int W, int cin, int cout, int taps,
                       int rows_per_chunk, void *stream);
-/* db[co] += sum_pixels dy[p][co] */
+/* db[co] += sum_pixels dy[p][co]; cout divides 256 */
 int gr_cnn_bias_grad(const void *dy, int ldd, int cd0, size_t npix, int cout, float *db,
                      void *stream);
 /* wt[ci][taps-1-tap][co] = w[co][tap][ci] (bf16) */
@@ -214,7 +214,8 @@ int gr_cnn_upsample(const void *x, int ldx, int cx0, void *y, int ldy, int cy0, 
 /* y[p][cy0+ch] = bf16(scale * x[p][ch]) for dense float x with c_in channels */
 int gr_cnn_pack_bf16(const float *x, int c_in, void *y, int ldy, int cy0, size_t npix,
                      float scale, void *stream);
-/* g[p][ch] = bf16(gscale (pred[p][ch] - target[p][ch])); *loss += sum of squares */
+/* g[p][ch] = bf16(gscale (pred[p][ch] - target[p][ch])); when loss is not
+ * NULL, *loss += sum of squares */
 int gr_cnn_mse_grad(const float *pred, int ldp, const float *target, int c, size_t npix, void *g,
                     int ldg, float gscale, float *loss, void *stream);
 /* Adam step (bias-corrected; wd adds wd * p to the gradient) on FP32 master

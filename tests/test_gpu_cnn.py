@@ -100,7 +100,7 @@ def test_conv_data_gradient_via_flipped_weights():
     assert rel(dx.cpu().double().numpy(), ref_dx) < 2e-6
 
 
-@pytest.mark.parametrize("taps,cin,cout,rpc", [(9, 32, 64, 3), (9, 16, 96, 1), (1, 32, 32, 5)])
+@pytest.mark.parametrize("taps,cin,cout,rpc", [(9, 32, 64, 3), (9, 16, 128, 1), (1, 32, 32, 5)])
 def test_conv_weight_and_bias_gradient(taps, cin, cout, rpc):
     rng = np.random.default_rng(5 + cin)
     n, H, W = 2, 6, 45
@@ -117,6 +117,9 @@ def test_conv_weight_and_bias_gradient(taps, cin, cout, rpc):
     _, ref_dw, ref_db = orc.conv_backward(x, w, dy)
     assert rel(dw.cpu().double().numpy().reshape(cout, taps, cin), ref_dw) < 2e-6
     assert rel(db.cpu().double().numpy(), ref_db) < 2e-6
+    with pytest.raises(ValueError):  # channel counts must divide 256
+        chk(lib().gr_cnn_bias_grad(dyd.data_ptr(), cout, 0, n * H * W, 96, db.data_ptr(),
+                                   stream()))
 
 
 def test_pool_upsample_relu_and_pack():
