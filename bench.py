@@ -196,11 +196,17 @@ def warmstart_main(args):
     if args.save_net:
         np.savez(args.save_net, params=net.p.cpu().numpy())
     r["losses"] = [(int(s), float(l)) for s, l in r["losses"]][-5:]
-    head = r["targets"]["plain_at_budget"]
+    # headline: the tightest tolerance both methods reach for every slice
+    # within the iteration cap (1e-3 when reached)
+    reached = [k for k in ("0.001", "0.003", "0.01")
+               if r["targets"][k]["time_ratio_all"] is not None]
+    tol = reached[0] if reached else None
     line = {
         "metric": "time-to-1e-3 rel err: warm-start vs plain PD",
-        "value": head["time_ratio_all"],
-        "unit": "x (plain time / warm-start time to plain's error at the iteration budget)",
+        "value": r["targets"][tol]["time_ratio_all"] if tol else None,
+        "unit": (f"x (plain time / warm-start time until every slice is within {tol} of u*; "
+                 "1e-3 not reached by either method within the cap)" if tol and tol != "0.001"
+                 else "x (plain time / warm-start time until every slice is within 1e-3 of u*)"),
         "n_gpus": 1, "steps": args.max_iters, "warmup": 0,
         "ms_per_step": r["iteration_ms_batch"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 solver, bf16 network (fp32 accumulate)",
