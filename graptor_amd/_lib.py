@@ -33,7 +33,8 @@ EXPORTS = (
     "gr_plan_iterate", "gr_plan_iterate_graph", "gr_plan_profile", "gr_plan_retune",
     "gr_cnn_conv_fwd", "gr_cnn_conv_wgrad", "gr_cnn_bias_grad", "gr_cnn_weight_flip",
     "gr_cnn_relu_bwd", "gr_cnn_pool", "gr_cnn_upsample", "gr_cnn_pack_bf16", "gr_cnn_mse_grad",
-    "gr_cnn_adam",
+    "gr_cnn_adam", "gr_tgv_dual_step", "gr_tgv_primal_step", "gr_csr_apply", "gr_prox",
+    "gr_sumsq_f64",
 )
 
 
@@ -119,6 +120,12 @@ def load(path: str | None = None):
             "gr_cnn_pack_bf16": ([V, I, V, I, I, S, F, V], I),
             "gr_cnn_mse_grad": ([V, I, V, I, S, V, I, F, V, V], I),
             "gr_cnn_adam": ([V, V, V, V, V, S, F, F, F, F, I, F, V], I),
+            "gr_tgv_dual_step": ([V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState), V], I),
+            "gr_tgv_primal_step": ([V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState), V],
+                                   I),
+            "gr_csr_apply": ([V, V, V, I, I, I, I, I, I, V, V, V], I),
+            "gr_prox": ([I, F, V, I, S, V, V, V, V], I),
+            "gr_sumsq_f64": ([V, S, PD, V], I),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

@@ -625,6 +625,30 @@ int gr_tgv_iterate(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_s
     return iterate_single(gg, P, n_iter, ku_on_last, stream_of(stream), nullptr, s);
 }
 
+int gr_tgv_dual_step(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
+                     void* stream)
+{
+    if (!valid_params(gg, p) || !s || !s->ubar || !s->v ||
+        (p->regularisation == GR_REG_TGV && (!s->pbar || !s->w))) {
+        set_error("tgv_dual_step: invalid parameters or missing state buffer");
+        return GR_ERR_INVALID;
+    }
+    const ProbDev P = make_prob(gg, p, s);
+    return launch_dual(P, stream_of(stream));
+}
+
+int gr_tgv_primal_step(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
+                       void* stream)
+{
+    if (!valid_params(gg, p) || !s || !s->u || !s->ubar || !s->v || !s->ktl ||
+        (p->regularisation == GR_REG_TGV && (!s->p || !s->pbar || !s->w))) {
+        set_error("tgv_primal_step: invalid parameters or missing state buffer");
+        return GR_ERR_INVALID;
+    }
+    const ProbDev P = make_prob(gg, p, s);
+    return launch_primal(P, stream_of(stream));
+}
+
 int gr_tgv_profile(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
                    int n_iter, double* ms_out, void* stream)
 {

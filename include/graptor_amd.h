@@ -138,6 +138,16 @@ int gr_tgv_iterate(const gr_geometry *g, const gr_tgv_params *prm,
 int gr_tgv_profile(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, double *ms_out, void *stream);
 
+/* One dual step (update_v/update_w + the coupled projections, 3D:615-660,
+ * 529-613) of the loop, alone: for solvers that compose the iteration from
+ * their own operators (the two-resolution solver, 3DS:1067-1072). */
+int gr_tgv_dual_step(const gr_geometry *g, const gr_tgv_params *prm, const gr_tgv_state *st,
+                     void *stream);
+/* One primal step with extrapolation (update_u/update_p/update_extra,
+ * 3DS:1098-1106) reading K^T lam from st->ktl. */
+int gr_tgv_primal_step(const gr_geometry *g, const gr_tgv_params *prm, const gr_tgv_state *st,
+                       void *stream);
+
 /* Primal energy at (u, p) reduced in FP64 on the device (no counterpart in
  * the reference's hot path; its Gap_TGV_KL 3D:882-927 is unused):
  *   KL: sum_c mu_c KL(R u_c, f_c) + TGV_{alpha/L}(u; p)
@@ -171,6 +181,26 @@ int gr_plan_profile(gr_plan *plan, int n_iter, double *ms_out, void *stream);
 /* Re-reads every geometry's tile shapes (gr_geometry_set_tuning) into the
  * plan; used by the tile autotuner.  Drops a captured graph. */
 int gr_plan_retune(gr_plan *plan, const gr_geometry *const *geoms);
+
+/* ------------------------------------------------------------------------
+ * Two-resolution solver pieces (radon_tgv_primal_dual_3dsubsample.py = 3DS:)
+ * ------------------------------------------------------------------------ */
+
+/* Replaces Sparsemultiplication 3DS:403-428: out[c][z][a][x] = sum over the
+ * CSR row (x + nx_out * z) of val * in[c][z'][a][x'] with column
+ * x' + nx_in * z'; rowstart has nx_out * nz_out + 1 entries, every column is
+ * < nx_in * nz_in (the caller guarantees it).  Applies the box-average
+ * interpolation between detector/slice grids and its adjoint. */
+int gr_csr_apply(const int *rowstart, const int *index, const float *val, int nx_out, int nz_out,
+                 int nx_in, int nz_in, int na, int nch, const float *in, float *out, void *stream);
+/* Replaces update_lambda_KL / _L2 3DS:229-256 as a standalone pass:
+ * lam = prox(lam + sigma_p ku) with per-channel mfac over nch blocks of
+ * per_channel bins. */
+int gr_prox(int discrepancy, float sigma_p, const float *mfac, int nch, size_t per_channel,
+            const float *ku, const float *f, float *lam, void *stream);
+/* *out = sum of x[i]^2 in FP64 (the norm of the power iteration 3DS:550).
+ * Synchronises the stream. */
+int gr_sumsq_f64(const float *x, size_t n, double *out, void *stream);
 
 /* ------------------------------------------------------------------------
  * Learned warm start: BF16 MFMA kernels of a small U-Net (no counterpart in
