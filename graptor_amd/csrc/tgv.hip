@@ -111,7 +111,8 @@ struct MemSrc {
     const float* __restrict__ ubar;
     const float* __restrict__ pbar;
     size_t cs;
-    __device__ __forceinline__ void own(size_t j, int, int, int, float& ub, float (&pb)[3]) const
+    __device__ __forceinline__ void own(int, size_t j, int, int, int, float& ub,
+                                        float (&pb)[3]) const
     {
         ub = ubar[j];
         if constexpr (TGV) {
@@ -119,8 +120,8 @@ struct MemSrc {
             for (int k = 0; k < DIM; ++k) pb[k] = pbar[j + k * cs];
         }
     }
-    __device__ __forceinline__ float ubn(size_t j, int, int, int) const { return ubar[j]; }
-    __device__ __forceinline__ void pbn(size_t j, int, int, int, float (&pb)[3]) const
+    __device__ __forceinline__ float ubn(int, size_t j, int, int, int) const { return ubar[j]; }
+    __device__ __forceinline__ void pbn(int, size_t j, int, int, int, float (&pb)[3]) const
     {
 #pragma unroll
         for (int k = 0; k < DIM; ++k) pb[k] = pbar[j + k * cs];
@@ -142,7 +143,7 @@ struct FusedSrc {
     int nx, ny, nz;
     size_t cs;
     float hinv, tau, norming;
-    __device__ __forceinline__ void own(size_t j, int x, int y, int z, float& ub,
+    __device__ __forceinline__ void own(int, size_t j, int x, int y, int z, float& ub,
                                         float (&pb)[3]) const
     {
         float un, pn[3];
@@ -155,14 +156,14 @@ struct FusedSrc {
             for (int k = 0; k < DIM; ++k) p_out[j + k * cs] = pn[k];
         }
     }
-    __device__ __forceinline__ float ubn(size_t j, int x, int y, int z) const
+    __device__ __forceinline__ float ubn(int, size_t j, int x, int y, int z) const
     {
         float un, ub, pn[3], pb[3];
         primal_point<DIM, TGV, true, false>(u, p, v, w, ktl[j], nx, ny, nz, cs, hinv, tau,
                                             norming, j, x, y, z, un, ub, pn, pb);
         return ub;
     }
-    __device__ __forceinline__ void pbn(size_t j, int x, int y, int z, float (&pb)[3]) const
+    __device__ __forceinline__ void pbn(int, size_t j, int x, int y, int z, float (&pb)[3]) const
     {
         float un, ub, pn[3];
         primal_point<DIM, TGV, false, true>(u, p, v, w, 0.f, nx, ny, nz, cs, hinv, tau, norming,
@@ -199,12 +200,12 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
         const size_t ic = i + c * chs;
         float u0;
         float p[3] = {0.f, 0.f, 0.f};
-        src.own(ic, x, y, z, u0, p);
+        src.own(c, ic, x, y, z, u0, p);
         float g[3];
-        g[0] = (x < nx - 1) ? src.ubn(ic + 1, x + 1, y, z) - u0 : 0.f;
-        g[1] = (y < ny - 1) ? src.ubn(ic + nx, x, y + 1, z) - u0 : 0.f;
+        g[0] = (x < nx - 1) ? src.ubn(c, ic + 1, x + 1, y, z) - u0 : 0.f;
+        g[1] = (y < ny - 1) ? src.ubn(c, ic + nx, x, y + 1, z) - u0 : 0.f;
         if constexpr (DIM == 3)
-            g[2] = (z < nz - 1) ? (src.ubn(ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
+            g[2] = (z < nz - 1) ? (src.ubn(c, ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             vv[c][k] = v[ic + k * cs] + sigma * (g[k] - p[k]);
@@ -214,18 +215,18 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
             float ax[3] = {0.f, 0.f, 0.f}, ay[3] = {0.f, 0.f, 0.f}, az[3] = {0.f, 0.f, 0.f};
             float q[3];
             if (x > 0) {
-                src.pbn(ic - 1, x - 1, y, z, q);
+                src.pbn(c, ic - 1, x - 1, y, z, q);
 #pragma unroll
                 for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
             }
             if (y > 0) {
-                src.pbn(ic - nx, x, y - 1, z, q);
+                src.pbn(c, ic - nx, x, y - 1, z, q);
 #pragma unroll
                 for (int k = 0; k < D; ++k) ay[k] = p[k] - q[k];
             }
             if constexpr (DIM == 3) {
                 if (z > 0) {
-                    src.pbn(ic - plane, x, y, z - 1, q);
+                    src.pbn(c, ic - plane, x, y, z - 1, q);
 #pragma unroll
                     for (int k = 0; k < D; ++k) az[k] = (p[k] - q[k]) * hinv;
                 }
@@ -557,9 +558,204 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void pd_fused_kernel(ProbDev P, F
     dual_core<DIM, CPL, TGV, MAXC>(P, x, y, z, src, B.v_in, B.w_in, B.v_out, B.w_out);
 }
 
+// LDS-tiled, z-marching form of the fused primal + next-dual step (3D).  A
+// work-group owns a 64 x 4 column of voxels over a run of FZ_RUN planes.
+// At plane z it has ubar and pbar of plane z in LDS (own values plus the +x/+y
+// halo of ubar and the -x/-y halo of pbar, evaluated by the first 136
+// threads), ubar of plane z + 1 and pbar of plane z - 1 in registers; it
+// evaluates the primal step of plane z + 1 (stored: the new u, ubar, p), then
+// the dual step of plane z from LDS and registers.  Each voxel's primal step
+// is stored once; the halo and the planes next to a run are recomputed.
+constexpr int FZ_RUN = 8;
+constexpr int FZ_W = DUAL_BX + 1;
+constexpr int FZ_H = DUAL_BY + 1;
+
+template <int MAXC, bool TGV>
+struct FzPlane {
+    float u[MAXC][FZ_H][FZ_W];              // [ly][lx] own, [ly][64] +x, [4][lx] +y
+    float p[MAXC][TGV ? 3 : 1][FZ_H][FZ_W]; // [ly+1][lx+1] own, [ly+1][0] -x, [0][lx+1] -y
+};
+
+template <int MAXC, bool TGV>
+struct FzSrc {
+    const FzPlane<MAXC, TGV>* sh;
+    float un[MAXC];       // ubar at (x, y, z + 1)
+    float pp[MAXC][3];    // pbar at (x, y, z - 1)
+    int lx, ly, x, z;
+    __device__ __forceinline__ void own(int c, size_t, int, int, int, float& ub,
+                                        float (&pb)[3]) const
+    {
+        ub = sh->u[c][ly][lx];
+        if constexpr (TGV) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pb[k] = sh->p[c][k][ly + 1][lx + 1];
+        }
+    }
+    __device__ __forceinline__ float ubn(int c, size_t, int xx, int, int zz) const
+    {
+        if (zz != z) return un[c];
+        return xx != x ? sh->u[c][ly][lx + 1] : sh->u[c][ly + 1][lx];
+    }
+    __device__ __forceinline__ void pbn(int c, size_t, int xx, int, int zz, float (&pb)[3]) const
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (zz != z) pb[k] = pp[c][k];
+            else if (xx != x) pb[k] = sh->p[c][TGV ? k : 0][ly + 1][lx];
+            else pb[k] = sh->p[c][TGV ? k : 0][ly][lx + 1];
+        }
+    }
+};
+
+template <int CPL, bool TGV, int MAXC>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void pd_fused_z_kernel(ProbDev P, FusedBufs B,
+                                                                      int tiles_x, int tiles_y)
+{
+    __shared__ FzPlane<MAXC, TGV> sh;
+    const int tid = threadIdx.x;
+    const int lx = tid & (DUAL_BX - 1), ly = tid / DUAL_BX;
+    int b = blockIdx.x;
+    const int tx = b % tiles_x;
+    b /= tiles_x;
+    const int ty = b % tiles_y;
+    const int zr = b / tiles_y;
+    const int x0 = tx * DUAL_BX, y0 = ty * DUAL_BY;
+    const int x = x0 + lx, y = y0 + ly;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nch = P.nch;
+    const int zs = zr * FZ_RUN, ze = min(nz, zs + FZ_RUN);
+    const bool in = x < nx && y < ny;
+    const size_t plane = (size_t)nx * ny, chs = plane * nz, cs = chs * nch;
+    const FusedSrc<3, TGV> fs{B.u_in, B.p_in, B.v_in, B.w_in, P.ktl, B.u_out, P.ubar,
+                              B.p_out, nx, ny, nz, cs, P.hinv, P.tau, P.norming};
+    auto at = [&](int xx, int yy, int zz, int c) {
+        return (size_t)c * chs + (size_t)zz * plane + (size_t)yy * nx + xx;
+    };
+    // halo of plane zz: ubar at +x / +y, pbar at -x / -y of the tile
+    auto halo = [&](int zz) {
+        if (tid < DUAL_BY) {
+            const int hx = x0 + DUAL_BX, hy = y0 + tid;
+            if (hx < nx && hy < ny) {
+#pragma unroll
+                for (int c = 0; c < MAXC; ++c)
+                    if (c < nch) sh.u[c][tid][DUAL_BX] = fs.ubn(c, at(hx, hy, zz, c), hx, hy, zz);
+            }
+        } else if (tid < DUAL_BY + DUAL_BX) {
+            const int r = tid - DUAL_BY, hx = x0 + r, hy = y0 + DUAL_BY;
+            if (hx < nx && hy < ny) {
+#pragma unroll
+                for (int c = 0; c < MAXC; ++c)
+                    if (c < nch) sh.u[c][DUAL_BY][r] = fs.ubn(c, at(hx, hy, zz, c), hx, hy, zz);
+            }
+        } else if (TGV && tid < 2 * DUAL_BY + DUAL_BX) {
+            const int r = tid - DUAL_BY - DUAL_BX, hx = x0 - 1, hy = y0 + r;
+            if (hx >= 0 && hy < ny) {
+#pragma unroll
+                for (int c = 0; c < MAXC; ++c) {
+                    if (c >= nch) break;
+                    float q[3];
+                    fs.pbn(c, at(hx, hy, zz, c), hx, hy, zz, q);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) sh.p[c][TGV ? k : 0][r + 1][0] = q[k];
+                }
+            }
+        } else if (TGV && tid < 2 * DUAL_BY + 2 * DUAL_BX) {
+            const int r = tid - 2 * DUAL_BY - DUAL_BX, hx = x0 + r, hy = y0 - 1;
+            if (hy >= 0 && hx < nx) {
+#pragma unroll
+                for (int c = 0; c < MAXC; ++c) {
+                    if (c >= nch) break;
+                    float q[3];
+                    fs.pbn(c, at(hx, hy, zz, c), hx, hy, zz, q);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) sh.p[c][TGV ? k : 0][0][r + 1] = q[k];
+                }
+            }
+        }
+    };
+
+    FzSrc<MAXC, TGV> src;
+    src.sh = &sh;
+    src.lx = lx;
+    src.ly = ly;
+    src.x = x;
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+        src.un[c] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) src.pp[c][k] = 0.f;
+    }
+    // prologue: primal step of plane zs (stored), pbar of plane zs - 1
+    if (in) {
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            if (c >= nch) break;
+            float ub, pb[3] = {0.f, 0.f, 0.f};
+            fs.own(c, at(x, y, zs, c), x, y, zs, ub, pb);
+            sh.u[c][ly][lx] = ub;
+            if constexpr (TGV) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sh.p[c][k][ly + 1][lx + 1] = pb[k];
+                if (zs > 0) fs.pbn(c, at(x, y, zs - 1, c), x, y, zs - 1, src.pp[c]);
+            }
+        }
+    }
+    halo(zs);
+    for (int z = zs; z < ze; ++z) {
+        float pn[MAXC][3];
+        const bool next_own = z + 1 < ze;
+        if (in && z + 1 < nz) {
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+                if (c >= nch) break;
+                const size_t j = at(x, y, z + 1, c);
+                if (next_own) fs.own(c, j, x, y, z + 1, src.un[c], pn[c]);
+                else src.un[c] = fs.ubn(c, j, x, y, z + 1);
+            }
+        }
+        __syncthreads();  // plane z complete in LDS
+        if (in) {
+            src.z = z;
+            dual_core<3, CPL, TGV, MAXC>(P, x, y, z, src, B.v_in, B.w_in, B.v_out, B.w_out);
+        }
+        __syncthreads();  // plane z no longer read
+        if (next_own) {
+            if (in) {
+#pragma unroll
+                for (int c = 0; c < MAXC; ++c) {
+                    if (c >= nch) break;
+                    sh.u[c][ly][lx] = src.un[c];
+                    if constexpr (TGV) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            src.pp[c][k] = sh.p[c][k][ly + 1][lx + 1];
+                            sh.p[c][k][ly + 1][lx + 1] = pn[c][k];
+                        }
+                    }
+                }
+            }
+            halo(z + 1);
+        }
+    }
+}
+
+template <int CPL, bool TGV, int MAXC>
+static void fused_z_launch(const ProbDev& P, const FusedBufs& B, hipStream_t st)
+{
+    const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
+    const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
+    const int runs = (P.nz + FZ_RUN - 1) / FZ_RUN;
+    hipLaunchKernelGGL((pd_fused_z_kernel<CPL, TGV, MAXC>),
+                       dim3((unsigned)(tiles_x * tiles_y * runs)), dim3(DUAL_BX * DUAL_BY), 0, st,
+                       P, B, tiles_x, tiles_y);
+}
+
 template <int DIM, int CPL, bool TGV, int MAXC>
 static void fused_launch(const ProbDev& P, const FusedBufs& B, hipStream_t st)
 {
+    if constexpr (DIM == 3) {
+        fused_z_launch<CPL, TGV, MAXC>(P, B, st);
+        return;
+    }
     const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
     const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
     hipLaunchKernelGGL((pd_fused_kernel<DIM, CPL, TGV, MAXC>),
