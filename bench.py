@@ -220,6 +220,106 @@ def warmstart_main(args):
     print(json.dumps(line, default=float), flush=True)
 
 
+def ellipse_main(args):
+    """Config: plain 2D single-channel TGV-L2 (128x128 ellipse phantom in a 183
+    grid, 90 angles in [-70, 70), 1% Gaussian noise, alpha (4, 1), mu 1, 500
+    iterations) on the GPU and, for the same data and L, on this host's CPU
+    with the FP32 C restatement of the reference kernels (all threads and one
+    thread); the relative L2 difference of the two results is reported."""
+    import numpy as np
+    import torch
+
+    from graptor_amd.phantoms import ellipse_problem_2d
+    from graptor_amd.solver import TgvProblem
+    from oracle import cref
+    from oracle import tgv_oracle as O
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    iters = 500
+    f, angles, ph, geom = ellipse_problem_2d(device=dev)
+    ofs = geom.ofs.T.copy()
+    L = O.normest_power(ofs, 183, 183, 183)
+    prob = TgvProblem(f, geom, (4.0, 1.0), [1.0], dim=2, coupling="uncorrelated",
+                      discrepancy="L2", L=L, device=dev)
+    fh = prob.f.cpu().numpy()
+    warm = prob.clone()
+    warm.iterate(50)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prob.iterate(iters)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    u = prob.u.cpu().numpy()
+    vox = prob.voxels()
+    # the same iterations replayed from a captured hipGraph (one-problem plan)
+    from graptor_amd.batch import BatchPlan
+    gprob = TgvProblem(f, geom, (4.0, 1.0), [1.0], dim=2, coupling="uncorrelated",
+                       discrepancy="L2", L=L, device=dev)
+    plan = BatchPlan([gprob])
+    plan.iterate_graph(50, block=50)
+    torch.cuda.synchronize()
+    gprob2 = TgvProblem(f, geom, (4.0, 1.0), [1.0], dim=2, coupling="uncorrelated",
+                        discrepancy="L2", L=L, device=dev)
+    plan2 = BatchPlan([gprob2])
+    plan2.iterate_graph(50, block=50)   # capture, then restart the state
+    for t in (gprob2.u, gprob2.ubar, gprob2.p, gprob2.pbar, gprob2.v, gprob2.w, gprob2.lam):
+        t.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan2.iterate_graph(iters, block=50)
+    torch.cuda.synchronize()
+    graph_s = time.perf_counter() - t0
+    graph_err = float(np.linalg.norm(gprob2.u.cpu().numpy() - u) / np.linalg.norm(u))
+    del plan, plan2
+    lib, arch, path = None, "x86-64-v3", ""
+    try:
+        tmp = tempfile.mkdtemp(prefix="cref_")
+        path = os.path.join(tmp, "libcpuref_native.so")
+        subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared", "-std=c11",
+                        os.path.join(ROOT, "oracle", "cpu_ref.c"), "-o", path, "-lm"],
+                       check=True, capture_output=True, timeout=120)
+        lib, arch = cref.load(path), "native"
+    except Exception:
+        lib = cref.load()
+    kw = dict(alpha=(4.0, 1.0), mu=[1.0], L=L, coupling="uncorrelated", discrepancy="L2",
+              dim=2, lib=lib)
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    r = cref.solve(fh, ofs, maxiter=iters, **kw)
+    cpu_s = time.perf_counter() - t0
+    env1 = dict(os.environ, OMP_NUM_THREADS="1")
+    code = ("import sys,time,numpy as np; sys.path.insert(0, %r); from oracle import cref; "
+            "d=np.load(%r); lib=cref.load(%r) if %r else cref.load(); t=time.perf_counter(); "
+            "cref.solve(d['f'], d['ofs'], alpha=(4.0,1.0), mu=[1.0], maxiter=%d, L=%r, "
+            "coupling='uncorrelated', discrepancy='L2', dim=2, lib=lib); "
+            "print(time.perf_counter()-t)")
+    npz = os.path.join(tempfile.mkdtemp(prefix="c1_"), "c1.npz")
+    np.savez(npz, f=fh, ofs=ofs)
+    one = subprocess.run([sys.executable, "-c", code % (ROOT, npz, path if arch == "native"
+                                                        else "", arch == "native", iters, L)],
+                         env=env1, capture_output=True, text=True, timeout=600)
+    cpu1_s = float(one.stdout.strip().splitlines()[-1]) if one.returncode == 0 else None
+    err = float(np.linalg.norm(u - r["u"]) / np.linalg.norm(r["u"]))
+    line = {
+        "metric": "voxel-iters/s (2D TGV-L2, 183x183, 90 angles)",
+        "value": vox * iters / gpu_s, "unit": "voxel-iters/s", "n_gpus": 1, "steps": iters,
+        "warmup": 50, "ms_per_step": gpu_s * 1e3 / iters, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipse phantom, 1% Gaussian noise)",
+        "config": {"workload": "2D single-channel TGV-L2, 128x128 ellipses in a 183 grid, "
+                               "90 angles in [-70,70), alpha (4,1), mu 1, 500 iterations"},
+        "cpu_baseline": {"value": vox * iters / cpu_s, "unit": "voxel-iters/s", "cores": threads,
+                         "kind": "port", "sample": f"the same 500 iterations, gcc -O3 {arch} "
+                                                   "OpenMP",
+                         "one_thread_value": (vox * iters / cpu1_s) if cpu1_s else None},
+        "rel_l2_gpu_vs_cpu": err,
+        "graph_replay": {"value": vox * iters / graph_s, "ms_per_step": graph_s * 1e3 / iters,
+                         "rel_l2_vs_eager": graph_err},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def unet_main(args):
     """The warm-start network's kernels alone: Adam steps of the 4-level
     U-Net (base 32) on a batch of 16 random 256 x 256 x 4 inputs and forward
@@ -290,7 +390,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--workload", default="coupled3d",
-                    choices=["coupled3d", "large3d", "batched2d", "warmstart", "unet"])
+                    choices=["coupled3d", "large3d", "batched2d", "warmstart", "unet",
+                             "ellipse2d"])
     ap.add_argument("--cap-iters", type=int, default=2000,
                     help="large3d: iteration cap of the run to a relative-change tolerance")
     ap.add_argument("--change-tol", type=float, default=1e-3)
@@ -320,6 +421,8 @@ def main():
         return warmstart_main(args)
     if args.workload == "unet":
         return unet_main(args)
+    if args.workload == "ellipse2d":
+        return ellipse_main(args)
 
     import numpy as np
     import torch

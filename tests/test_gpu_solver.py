@@ -242,3 +242,59 @@ def test_fused_primal_dual_z_runs_and_partial_tiles(coupling, reg):
                 continue
             err = rel(a.cpu().numpy(), b.cpu().numpy())
             assert err <= 1e-6, (n, name, err)
+
+
+@pytest.mark.parametrize("dim,C", [(3, 6), (2, 8)])
+def test_many_channel_parity(dim, C):
+    """Channel counts above 4 (the 8-channel register layout of the dual step)."""
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    print(f"many channels dim={dim} C={C}", flush=True)
+    Nz = 1 if dim == 2 else 3
+    _, f, ofs = make_problem(53, C, Nz, 32, 16, poisson=50.0)
+    f = f.astype(np.float32)
+    N = 32
+    ang = np.deg2rad(np.linspace(-70, 70, 16, endpoint=False)) + np.pi / 2
+    L = O.normest_power(ofs, N, N, N)
+    g = RadonGeometry((N, N), ang, n_detectors=N)
+    mu = list(np.linspace(1.0, 0.1, C))
+    prob = TgvProblem(f, g, (4, 1), mu, dim=dim, coupling="frobenius", discrepancy="KL", L=L)
+    prob.iterate(40)
+    torch.cuda.synchronize()
+    r32 = cref.solve(f, ofs, alpha=(4, 1), mu=mu, maxiter=40, L=L, coupling="frobenius",
+                     discrepancy="KL", dim=dim)
+    e = rel(prob.u.cpu().numpy(), r32["u"])
+    print(f"  rel L2 vs FP32 restatement {e:.2e}", flush=True)
+    assert e <= 1e-4
+
+
+def test_ellipse_2d_l2_configuration_parity():
+    """The 2D single-channel TGV-L2 configuration (128x128 ellipse phantom in a
+    183 grid, 90 angles in [-70, 70), 1% Gaussian noise, alpha (4, 1), mu 1,
+    500 iterations) through the drop-in entry, against the FP32 restatement
+    and the FP64 oracle with the same L."""
+    from graptor_amd.phantoms import ellipse_problem_2d
+    from graptor_amd.radon_tgv_primal_dual_2d import Reconstructionapproach2d
+    print("ellipse 2D L2 500 iterations", flush=True)
+    f, angles, ph, geom = ellipse_problem_2d()
+    fh = f.cpu().numpy()                               # (1, 1, 90, 183)
+    ofs = geom.ofs.T.copy()
+    fh = fh * O.feasible_mask(ofs, 183, 183, 183)[None, None]
+    L = O.normest_power(ofs, 183, 183, 183)
+    sol, sinos = Reconstructionapproach2d([fh[0, 0]], angles, (4.0, 1.0), [1.0], 500, None,
+                                          [0, 0], "L2", "Uncorrelated 2D Reconstruction", "TGV",
+                                          [0.0, 1.0, 0], L=L)
+    u = sol[0].T                                       # [x, y] -> [y, x]
+    r32 = cref.solve(fh.astype(np.float32), ofs, alpha=(4.0, 1.0), mu=[1.0], maxiter=500, L=L,
+                     coupling="uncorrelated", discrepancy="L2", dim=2)
+    e32 = rel(u, r32["u"][0, 0])
+    # the FP64 oracle (NumPy) on the first 20 iterations
+    s20, _ = Reconstructionapproach2d([fh[0, 0]], angles, (4.0, 1.0), [1.0], 20, None, [0, 0],
+                                      "L2", "Uncorrelated 2D Reconstruction", "TGV",
+                                      [0.0, 1.0, 0], L=L)
+    r64 = O.solve(fh.astype(np.float64), ofs, (4.0, 1.0), [1.0], 20, L, "uncorrelated", "L2",
+                  dim=2)
+    e64 = rel(s20[0].T, r64["u"][0, 0])
+    print(f"  rel L2 vs FP32 restatement (500 it) {e32:.2e}, vs FP64 oracle (20 it) {e64:.2e}",
+          flush=True)
+    assert e32 <= 1e-4 and e64 <= 1e-4
