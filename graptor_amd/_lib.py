@@ -69,12 +69,13 @@ _lock = threading.Lock()
 
 
 def load(path: str | None = None):
-    """Load the library once and declare its signatures."""
+    """Load the library once and declare its signatures (GRAPTOR_AMD_LIB, if
+    set, names another build of it)."""
     global _lib
     with _lock:
         if _lib is not None and path is None:
             return _lib
-        p = path or LIB_PATH
+        p = path or os.environ.get("GRAPTOR_AMD_LIB") or LIB_PATH
         if not os.path.exists(p):
             raise ImportError(
                 f"libgraptor_amd.so not found at {p}; build it with "

@@ -102,7 +102,15 @@ constexpr int FWD_PADL = 4;          // zero candidates left of position 0
 constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_TX = 64;           // backprojection tile width (one wave)
 constexpr int ADJ_MAXR = 8;          // rows per thread (tile_y <= 32)
-constexpr int DUAL_BX = 64, DUAL_BY = 4;
+// stencil tile (threads per work-group = DUAL_BX * DUAL_BY); the macros
+// allow other shapes to be built for measurement
+#ifndef GR_DUAL_BX
+#define GR_DUAL_BX 64
+#endif
+#ifndef GR_DUAL_BY
+#define GR_DUAL_BY 4
+#endif
+constexpr int DUAL_BX = GR_DUAL_BX, DUAL_BY = GR_DUAL_BY;
 
 // Kernel-side sizes derived from a problem and its tuning.
 int fwd_pitch_of(int nx, int ny);

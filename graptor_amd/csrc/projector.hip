@@ -107,12 +107,18 @@ __device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int
 
     const float cand_hi = (float)len_cand;
     const size_t slice_stride = (size_t)nx * ny;
+    // staging walk: element e = tid + j * FWD_THREADS of a band, split into
+    // (outer, inner) = (e / n_in, e % n_in) by stepping rather than dividing
+    // (n_in = candidates per line for rows, band for columns)
+    const int n_in = I.cls == 1 ? len_cand : band;
+    const int n_out = I.cls == 1 ? band : len_cand;
+    const int st_out0 = tid / n_in, st_in0 = tid - st_out0 * n_in;
+    const int st_dout = FWD_THREADS / n_in, st_din = FWD_THREADS - st_dout * n_in;
     for (int l0 = 0; l0 < len_step; l0 += band) {
         __syncthreads();
         // stage lines [l0, l0+band) of K slices, candidates contiguous
         if (I.cls == 1) {  // lines are rows, candidates are x
-            for (int e = tid; e < band * len_cand; e += FWD_THREADS) {
-                const int li = e / len_cand, x = e - li * len_cand;
+            for (int li = st_out0, x = st_in0; li < n_out;) {
                 const int y = l0 + li;
                 float v[K];
 #pragma unroll
@@ -123,10 +129,15 @@ __device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int
                                : 0.f;
                 }
                 lds_store<K>(sh, li * pitch + FWD_PADL + x, v);
+                x += st_din;
+                li += st_dout;
+                if (x >= n_in) {
+                    x -= n_in;
+                    ++li;
+                }
             }
         } else {  // lines are columns, candidates are y
-            for (int e = tid; e < band * len_cand; e += FWD_THREADS) {
-                const int y = e / band, li = e - y * band;
+            for (int y = st_out0, li = st_in0; y < n_out;) {
                 const int x = l0 + li;
                 float v[K];
 #pragma unroll
@@ -137,6 +148,12 @@ __device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int
                                : 0.f;
                 }
                 lds_store<K>(sh, li * pitch + FWD_PADL + y, v);
+                li += st_din;
+                y += st_dout;
+                if (li >= n_in) {
+                    li -= n_in;
+                    ++y;
+                }
             }
         }
         __syncthreads();
