@@ -340,6 +340,8 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
         for (int k = 0; k < K; ++k) acc[r][k] = 0.f;
 
     const size_t sino_slice = (size_t)na * nd;
+    const int sg_a0 = tid / segw, sg_b0 = tid - sg_a0 * segw;
+    const int sg_da = ADJ_THREADS / segw, sg_db = ADJ_THREADS - sg_da * segw;
     for (int a0 = 0; a0 < na; a0 += achunk) {
         const int A = min(achunk, na - a0);
         __syncthreads();
@@ -353,8 +355,9 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
             lo_s[tid] = lo;
         }
         __syncthreads();
-        for (int e = tid; e < A * segw; e += ADJ_THREADS) {
-            const int al = e / segw, b = e - al * segw;
+        // element e = tid + j * ADJ_THREADS as (angle, bin) = (e / segw, e % segw),
+        // stepped rather than divided
+        for (int al = sg_a0, b = sg_b0; al < A;) {
             const int gb = lo_s[al] + b;
             float v[K];
 #pragma unroll
@@ -365,6 +368,12 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
                            : 0.f;
             }
             lds_store<K>(seg, al * segw + b, v);
+            b += sg_db;
+            al += sg_da;
+            if (b >= segw) {
+                b -= segw;
+                ++al;
+            }
         }
         __syncthreads();
         for (int al = 0; al < A; ++al) {
