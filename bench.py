@@ -395,8 +395,8 @@ def main():
     ap.add_argument("--cap-iters", type=int, default=2000,
                     help="large3d: iteration cap of the run to a relative-change tolerance")
     ap.add_argument("--change-tol", type=float, default=1e-3)
-    ap.add_argument("--train-pairs", type=int, default=8192)
-    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--train-pairs", type=int, default=20000)
+    ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--eval-slices", type=int, default=256)
     ap.add_argument("--target-iters", type=int, default=2500)
     ap.add_argument("--ustar-iters", type=int, default=20000)

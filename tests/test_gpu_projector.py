@@ -21,6 +21,8 @@ CASES = [
     (183, 183, 183, angles_deg(-70, 70, 90), 1),
     (40, 40, 57, np.array([0.0, np.pi / 4, np.pi / 2, 3 * np.pi / 4, np.pi, -0.3, 2.0]), 4),
     (256, 256, 256, angles_deg(-75, 75, 60), 8),
+    (70, 50, 90, np.linspace(0.0, np.pi, 33, endpoint=False), 11),   # 8 + 3 slices
+    (96, 96, 96, angles_deg(-75, 75, 45), 16),
 ]
 
 
