@@ -161,21 +161,24 @@ __device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int
 #pragma unroll
         for (int it = 0; it < MI; ++it) {
             if (tid + it * FWD_THREADS >= nitems) continue;
-            for (int st = 0; st < band; ++st) {
-                const float xs = (float)(l0 + st);
+            float xs = (float)l0;  // exact integer-valued step coordinate
+            for (int st = 0; st < band; ++st, xs += 1.f) {
                 const float r = fmaf(xs, ms[it], of[it]) - fi[it];
                 float bf = ceilf(fmaf(-r, iv[it], -hh[it]));
-                bf = fminf(fmaxf(bf, -3.f), cand_hi);
+                // clamp into [-3, len_cand] (bf is never NaN: finite operands)
+                bf = __builtin_amdgcn_fmed3f(bf, -3.f, cand_hi);
                 const float d0 = fmaf(bf, mc[it], r);
                 const float d1 = d0 + mc[it];
                 const float d2 = d1 + mc[it];
                 const float w0 = fmaxf(0.f, 1.f - fabsf(d0));
                 const float w1 = fmaxf(0.f, 1.f - fabsf(d1));
                 const float w2 = fmaxf(0.f, 1.f - fabsf(d2));
-                const int pos = st * pitch + FWD_PADL + (int)bf;
-                const Pack<K> v0 = lds_load<K>(sh, pos);
-                const Pack<K> v1 = lds_load<K>(sh, pos + 1);
-                const Pack<K> v2 = lds_load<K>(sh, pos + 2);
+                // line base is uniform across the wave; the candidate offset is per lane
+                const float* line = sh + K * (st * pitch + FWD_PADL);
+                const int c0 = (int)bf;
+                const Pack<K> v0 = lds_load<K>(line, c0);
+                const Pack<K> v1 = lds_load<K>(line, c0 + 1);
+                const Pack<K> v2 = lds_load<K>(line, c0 + 2);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     float a = fmaf(w0, v0.v[k], acc[it][k]);
