@@ -569,6 +569,8 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void pd_fused_kernel(ProbDev P, F
 constexpr int FZ_RUN = 8;
 constexpr int FZ_W = DUAL_BX + 1;
 constexpr int FZ_H = DUAL_BY + 1;
+static_assert(2 * (DUAL_BX + DUAL_BY) <= DUAL_BX * DUAL_BY,
+              "the halo of a tile needs one thread per halo entry");
 
 template <int MAXC, bool TGV>
 struct FzPlane {
