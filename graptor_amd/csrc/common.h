@@ -112,6 +112,28 @@ constexpr int ADJ_MAXR = 8;          // rows per thread (tile_y <= 32)
 #endif
 constexpr int DUAL_BX = GR_DUAL_BX, DUAL_BY = GR_DUAL_BY;
 
+#ifndef GR_DUAL_GROUP
+#define GR_DUAL_GROUP 8
+#endif
+#ifndef GR_PRIMAL_GROUP
+#define GR_PRIMAL_GROUP 4
+#endif
+constexpr int DUAL_GROUP = GR_DUAL_GROUP, PRIMAL_GROUP = GR_PRIMAL_GROUP;
+// Work id of a single-problem stencil work-group.  Hardware dispatch deals
+// work-groups round-robin over the 8 XCDs; here runs of G consecutive tiles
+// (neighbours in x, then y) are kept on one XCD so that the cache lines their
+// neighbour reads share are fetched into one L2, while the 8 XCDs still work
+// on adjacent runs (dense, concurrently active addresses).  Grids that are
+// not a multiple of 8 runs keep hardware order.
+template <int G>
+__device__ __forceinline__ int stencil_block()
+{
+    const int b = blockIdx.x;
+    if (G <= 1 || gridDim.x % (8 * G) != 0) return b;
+    const int xcd = b & 7, local = b >> 3;
+    return ((local / G) * 8 + xcd) * G + local % G;
+}
+
 // Kernel-side sizes derived from a problem and its tuning.
 int fwd_pitch_of(int nx, int ny);
 int fwd_table_choice(const Geometry& g, long long groups);

@@ -88,16 +88,23 @@ __device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int
     const int s0 = I.s0;
     const float* __restrict__ img = P.ubar;
 
+    // Per item, along the ray at stepping coordinate xs: the offset of the
+    // candidate-axis origin from bin i is r = xs * mstep + (off - i); the
+    // first candidate whose hat weight can be positive is
+    // ceil(-r / mcand - hc) = ceil(xs * ta + tb).
     float acc[MI][K];
-    float fi[MI], ms[MI], mc[MI], of[MI], iv[MI], hh[MI];
+    float ms[MI], mc[MI], ro[MI], ta[MI], tb[MI];
 #pragma unroll
     for (int it = 0; it < MI; ++it) {
         const int item = tid + it * FWD_THREADS;
         const int a = item < nitems ? item / I.nb : 0;
         const FwdAngle A = P.fwd[I.a0 + a];
-        fi[it] = (float)(I.b0 + (item < nitems ? item - a * I.nb : 0));
-        ms[it] = A.mstep; mc[it] = A.mcand; of[it] = A.off;
-        iv[it] = A.invc;  hh[it] = A.hc;
+        const float fi = (float)(I.b0 + (item < nitems ? item - a * I.nb : 0));
+        ms[it] = A.mstep;
+        mc[it] = A.mcand;
+        ro[it] = A.off - fi;
+        ta[it] = -A.mstep * A.invc;
+        tb[it] = fmaf(-ro[it], A.invc, -A.hc);
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[it][k] = 0.f;
     }
@@ -163,16 +170,17 @@ __device__ __forceinline__ void fwd_body(const ProbDev& P, const FwdItem& I, int
             if (tid + it * FWD_THREADS >= nitems) continue;
             float xs = (float)l0;  // exact integer-valued step coordinate
             for (int st = 0; st < band; ++st, xs += 1.f) {
-                const float r = fmaf(xs, ms[it], of[it]) - fi[it];
-                float bf = ceilf(fmaf(-r, iv[it], -hh[it]));
+                const float r = fmaf(xs, ms[it], ro[it]);
+                float bf = ceilf(fmaf(xs, ta[it], tb[it]));
                 // clamp into [-3, len_cand] (bf is never NaN: finite operands)
                 bf = __builtin_amdgcn_fmed3f(bf, -3.f, cand_hi);
                 const float d0 = fmaf(bf, mc[it], r);
                 const float d1 = d0 + mc[it];
                 const float d2 = d1 + mc[it];
-                const float w0 = fmaxf(0.f, 1.f - fabsf(d0));
-                const float w1 = fmaxf(0.f, 1.f - fabsf(d1));
-                const float w2 = fmaxf(0.f, 1.f - fabsf(d2));
+                // hat weights max(0, 1 - |d|) (1 - |d| <= 1, so a clamp to [0, 1])
+                const float w0 = __builtin_amdgcn_fmed3f(1.f - fabsf(d0), 0.f, 1.f);
+                const float w1 = __builtin_amdgcn_fmed3f(1.f - fabsf(d1), 0.f, 1.f);
+                const float w2 = __builtin_amdgcn_fmed3f(1.f - fabsf(d2), 0.f, 1.f);
                 // line base is uniform across the wave; the candidate offset is per lane
                 const float* line = sh + K * (st * pitch + FWD_PADL);
                 const int c0 = (int)bf;
@@ -297,7 +305,7 @@ template <int DIM, bool TGV>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_single_kernel(ProbDev P, int tiles_x,
                                                                          int tiles_y)
 {
-    const int b = blockIdx.x;
+    const int b = stencil_block<PRIMAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
     primal_body<DIM, TGV>(P, I);
@@ -386,11 +394,12 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
 #pragma unroll
             for (int r = 0; r < ADJ_MAXR; ++r) {
                 if (r >= R) break;
+                // t >= 1 over the whole tile (the segment starts at
+                // floor(min t) - 1), so truncation is the floor
                 const float t = fmaf((float)(y0 + wv + 4 * r), pa.y, tb);
-                const float fl = floorf(t);
-                const float p = t - fl;
+                const float p = __builtin_amdgcn_fractf(t);
                 const float q = 1.f - p;
-                const int i0 = base + (int)fl;
+                const int i0 = base + (int)t;
                 const Pack<K> v0 = lds_load<K>(seg, i0);
                 const Pack<K> v1 = lds_load<K>(seg, i0 + 1);
 #pragma unroll

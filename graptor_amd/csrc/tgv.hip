@@ -365,14 +365,14 @@ __device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
     dual_core<DIM, CPL, TGV, MAXC>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
 }
 
-// Work order: x tiles fastest, then rows, then planes (plane-major order
-// re-reads fewer neighbour planes but spreads concurrent work-groups over
-// fewer HBM channels, and measured slower).
+// Work order: x tiles fastest, then rows, then planes, with runs of
+// DUAL_GROUP tiles kept on one XCD (plane-major orders, z-fastest orders and
+// giving each XCD a contiguous eighth of the volume all measured slower).
 template <int DIM, int CPL, bool TGV, int MAXC>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_single_kernel(ProbDev P, int tiles_x,
                                                                        int tiles_y)
 {
-    const int b = blockIdx.x;
+    const int b = stencil_block<DUAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
     dual_body<DIM, CPL, TGV, MAXC>(P, I);
