@@ -233,6 +233,13 @@ struct gr_plan {
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_block = 0;
+    // the launches of one phase touch disjoint problems: up to PLAN_LANES of
+    // them run side by side, forked from the caller's stream and joined back
+    // (also inside a captured graph, as parallel branches)
+    static constexpr int PLAN_LANES = 4;
+    hipStream_t side[PLAN_LANES - 1] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[PLAN_LANES - 1] = {};
 };
 
 namespace {
@@ -374,25 +381,55 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
     return rc;
 }
 
+int plan_lanes(gr_plan* pl)
+{
+    if (pl->fork_ev) return GR_OK;
+    for (auto& s : pl->side) GR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (auto& e : pl->join_ev) GR_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    GR_HIP(hipEventCreateWithFlags(&pl->fork_ev, hipEventDisableTiming));
+    return GR_OK;
+}
+
 int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
 {
+    const size_t nl = phase == 0 ? pl->dual.size()
+                    : phase == 1 ? pl->fwd.size()
+                    : phase == 2 ? pl->adj.size()
+                                 : pl->primal.size();
+    const int lanes = (int)std::min<size_t>(nl, gr_plan::PLAN_LANES);
+    if (lanes > 1) {
+        int rc = plan_lanes(pl);
+        if (rc) return rc;
+        GR_HIP(hipEventRecord(pl->fork_ev, st));
+        for (int j = 1; j < lanes; ++j) GR_HIP(hipStreamWaitEvent(pl->side[j - 1], pl->fork_ev, 0));
+    }
     int rc = GR_OK;
-    if (phase == 0) {
-        for (auto& l : pl->dual)
-            if (!rc) rc = launch_dual_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim,
-                                            l.coupling, l.tgv != 0, l.maxc, st);
-    } else if (phase == 1) {
-        for (auto& l : pl->fwd)
-            if (!rc) rc = launch_fwd_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.mi,
-                                           l.lds, fwd_mode, st);
-    } else if (phase == 2) {
-        for (auto& l : pl->adj)
-            if (!rc) rc = launch_adj_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.dim,
-                                           l.tgv != 0, l.fuse != 0, l.lds, st);
-    } else {
-        for (auto& l : pl->primal)
-            if (!rc) rc = launch_primal_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim,
-                                              l.tgv != 0, st);
+    for (size_t i = 0; i < nl && !rc; ++i) {
+        const int lane = (int)(i % (size_t)std::max(lanes, 1));
+        const hipStream_t s = lane == 0 ? st : pl->side[lane - 1];
+        if (phase == 0) {
+            const auto& l = pl->dual[i];
+            rc = launch_dual_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim, l.coupling,
+                                   l.tgv != 0, l.maxc, s);
+        } else if (phase == 1) {
+            const auto& l = pl->fwd[i];
+            rc = launch_fwd_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.mi, l.lds,
+                                  fwd_mode, s);
+        } else if (phase == 2) {
+            const auto& l = pl->adj[i];
+            rc = launch_adj_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.dim,
+                                  l.tgv != 0, l.fuse != 0, l.lds, s);
+        } else {
+            const auto& l = pl->primal[i];
+            rc = launch_primal_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim,
+                                     l.tgv != 0, s);
+        }
+    }
+    // join every forked lane, also after a failed launch (a capture must end joined)
+    for (int j = 1; j < lanes; ++j) {
+        hipError_t e = hipEventRecord(pl->join_ev[j - 1], pl->side[j - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, pl->join_ev[j - 1], 0);
+        if (e != hipSuccess && !rc) rc = hip_status(e, "plan: joining side streams");
     }
     return rc;
 }
@@ -727,6 +764,9 @@ int gr_plan_destroy(gr_plan* pl)
     if (!pl) return GR_OK;
     plan_free_device(pl);
     if (pl->cap_stream) (void)hipStreamDestroy(pl->cap_stream);
+    for (auto s : pl->side) if (s) (void)hipStreamDestroy(s);
+    for (auto e : pl->join_ev) if (e) (void)hipEventDestroy(e);
+    if (pl->fork_ev) (void)hipEventDestroy(pl->fork_ev);
     delete pl;
     return GR_OK;
 }
