@@ -125,6 +125,36 @@ ProbDev make_prob(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
     return P;
 }
 
+// A side stream and fork/join events per host thread and device, for
+// running the dual step and the p half of the primal step beside the
+// projections of the same iteration.  Created on first use outside any
+// stream capture and kept until the process ends.
+struct Lane {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, dual_done = nullptr, join = nullptr;
+};
+
+static Lane* lane_for(hipStream_t st)
+{
+    static thread_local Lane lanes[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    Lane& L = lanes[dev];
+    if (L.join) return &L;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return nullptr;
+    if (hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&L.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&L.dual_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&L.join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        L = Lane{};
+        return nullptr;
+    }
+    return &L;
+}
+
 // One launch with an optional HIP event pair accumulated into *acc.
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -189,6 +219,32 @@ int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_o
     int it = 0;
     // unfused iterations: all of them, or one to make the fused count even
     const int plain = fused ? ((n_iter - 1) % 2) : n_iter;
+    // Two streams per iteration k (untimed runs):
+    //   side: dual_k, then the p half of primal_k (needs only v, w)
+    //   main: forward + prox_k, backprojection_k, then (after dual_k) the u
+    //         half of primal_k
+    // dual_k+1 waits for the u half of primal_k (it reads ubar_k+1); the
+    // halves write disjoint arrays, the projections only read ubar and lam.
+    Lane* lane = (!ms_out && !P.fuse_primal) ? lane_for(st) : nullptr;
+    for (; lane && it < plain && rc == GR_OK; ++it) {
+        GR_HIP(hipEventRecord(lane->fork, st));
+        GR_HIP(hipStreamWaitEvent(lane->side, lane->fork, 0));
+        rc = launch_dual(P, lane->side);
+        if (!rc) {
+            GR_HIP(hipEventRecord(lane->dual_done, lane->side));
+            rc = launch_primal_part(P, PRIMAL_P, lane->side);
+        }
+        if (!rc) rc = launch_fwd(P, gg->g, nsl, fwd_mode(it), st);
+        if (!rc) rc = launch_adj(P, nsl, false, st);
+        if (!rc) {
+            GR_HIP(hipStreamWaitEvent(st, lane->dual_done, 0));
+            rc = launch_primal_part(P, PRIMAL_U, st);
+        }
+    }
+    if (lane) {  // the caller's stream ends after everything of the side stream
+        GR_HIP(hipEventRecord(lane->join, lane->side));
+        GR_HIP(hipStreamWaitEvent(st, lane->join, 0));
+    }
     for (; it < plain && rc == GR_OK; ++it) {
         rc = t.run(&ms[0], [&] { return launch_dual(P, st); });
         if (!rc) rc = t.run(&ms[1], [&] { return launch_fwd(P, gg->g, nsl, fwd_mode(it), st); });

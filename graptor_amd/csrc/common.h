@@ -152,6 +152,9 @@ int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st);
 int launch_dual(const ProbDev& P, hipStream_t st);
 int launch_fused_pd(const ProbDev& P, const FusedBufs& B, hipStream_t st);
 int launch_primal(const ProbDev& P, hipStream_t st);
+// the u half (PRIMAL_U), the p half (PRIMAL_P) or both (PRIMAL_UP)
+enum { PRIMAL_U = 1, PRIMAL_P = 2, PRIMAL_UP = 3 };
+int launch_primal_part(const ProbDev& P, int part, hipStream_t st);
 
 // ---- launchers (batches: descriptor table + item table) -------------------
 int launch_fwd_batch(const ProbDev* probs_dev, const FwdItem* items, int nitems, int K,
