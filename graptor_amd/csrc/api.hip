@@ -127,8 +127,9 @@ ProbDev make_prob(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
 
 // A side stream and fork/join events per host thread and device, for
 // running the dual step and the p half of the primal step beside the
-// projections of the same iteration.  Created on first use outside any
-// stream capture and kept until the process ends.
+// projections of the same iteration.  Created on first use and kept until
+// the process ends.  Not used while the caller's stream is being captured
+// (captured iterations keep the one-stream order).
 struct Lane {
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, dual_done = nullptr, join = nullptr;
@@ -137,13 +138,13 @@ struct Lane {
 static Lane* lane_for(hipStream_t st)
 {
     static thread_local Lane lanes[64];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return nullptr;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
     Lane& L = lanes[dev];
     if (L.join) return &L;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
-        return nullptr;
     if (hipStreamCreateWithFlags(&L.side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&L.fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&L.dual_done, hipEventDisableTiming) != hipSuccess ||
