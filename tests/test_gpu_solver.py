@@ -184,6 +184,42 @@ def test_fused_primal_matches_unfused(dim):
     assert rel(out[1][1], out[0][1]) <= 1e-6
 
 
+@pytest.mark.parametrize("combo", COMBOS[:4] + COMBOS[4:5],
+                         ids=["-".join(map(str, c)) for c in COMBOS[:4] + COMBOS[4:5]])
+def test_two_stream_iteration_matches_one_stream(combo):
+    """Untimed iterations run the dual step and the p half of the primal step
+    on a side stream; the profiled form runs everything on the caller's
+    stream with the primal step as one kernel.  Both must give the same bits,
+    and work queued on the caller's stream right after the call (no host
+    synchronisation) must see the side stream's results."""
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    dim, coupling, disc, reg = combo
+    print(f"two streams vs one {combo}", flush=True)
+    _, f, ofs = _problem(dim, C=3, N=48, na=20)
+    C, Nz, na, N = f.shape
+    ang = np.deg2rad(np.linspace(-70, 70, na, endpoint=False)) + np.pi / 2
+    L = O.normest_power(ofs, N, N, N)
+    g = RadonGeometry((N, N), ang, n_detectors=N)
+    kw = dict(dim=dim, coupling=coupling, discrepancy=disc, regularisation=reg, L=L)
+    two = TgvProblem(f, g, (4, 1), [1.0, 0.3, 0.1], **kw)
+    one = TgvProblem(f, g, (4, 1), [1.0, 0.3, 0.1], **kw)
+    for n in (1, 7, 12):
+        two.iterate(n)
+        # queued on the caller's stream before any host synchronisation
+        early = {k: getattr(two, k).sum() for k in ("u", "p", "v", "w")
+                 if getattr(two, k) is not None}
+        one.profile(n)
+        torch.cuda.synchronize()
+        for k, t in early.items():
+            assert float(t) == float(getattr(two, k).sum()), (n, k)
+        for name in ("u", "ubar", "p", "pbar", "v", "w", "lam"):
+            a, b = getattr(two, name), getattr(one, name)
+            if b is None:
+                continue
+            assert torch.equal(a, b), (n, name)
+
+
 @pytest.mark.parametrize("combo", COMBOS, ids=["-".join(map(str, c)) for c in COMBOS])
 def test_fused_primal_dual_kernel_matches_separate_kernels(combo):
     """The primal step of each iteration fused with the next dual step (two
