@@ -374,7 +374,7 @@ def unet_main(args):
                                "(training) / 32 (inference), 256x256x4"},
         "roofline": {"bound": "mfma", "achieved": infer_tf, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": infer_tf / 2500.0, "traffic": None,
-                     "kernel": "forward pass (conv_fwd_kernel and the elementwise kernels)"},
+                     "kernel": "forward pass (conv_tile_kernel and the elementwise kernels)"},
         "train_tflops": train_tf, "inference_ms_batch32": infer_ms,
     }
     print(json.dumps(line), flush=True)

@@ -49,73 +49,181 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 // ---------------------------------------------------------------------------
-// Convolution forward as implicit GEMM: M = pixels (32 consecutive x of one
-// row per wave tile), N = output channels (32 per wave tile), K = taps x ci.
-// A work-group of 4 waves covers 64 pixels x 64 channels (2 x 2 wave tiles).
-// Requires cin % 16 == 0, cout % 32 == 0, channel offsets and strides
-// multiples of 8; rows of any width (partial wave tiles are masked).
+// Convolution forward, LDS-tiled: a work-group of 4 waves computes MP
+// consecutive pixels of one output row x NC output channels.  Per chunk of
+// KC input channels it stages the R input rows the taps touch (MP + 2
+// pixels, zero outside the image and beyond cin) and the NC x taps x KC
+// weights in LDS; each wave then runs TM x TN MFMA tiles per 16-wide k step,
+// reusing every A fragment TN times and every B fragment TM times.  Strides
+// in LDS are padded (KC + 8 halves per pixel, taps * KC + 8 per channel) so
+// that the 16-byte fragment reads of 16 consecutive lanes hit distinct banks.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void conv_fwd_kernel(
-    const uint16_t* __restrict__ x, int ldx, int cx0, const uint16_t* __restrict__ w,
-    const float* __restrict__ bias, void* __restrict__ y, int ldy, int cy0, int n, int H, int W,
-    int cin, int cout, int taps, int relu, int out_f32)
+template <int WAVES_M_, int TM_, int TN_, int KC_, int TAPS_>
+struct ConvTile {
+    static constexpr int WAVES_M = WAVES_M_, TM = TM_, TN = TN_, KC = KC_, TAPS = TAPS_;
+    static constexpr int WAVES_N = 4 / WAVES_M;
+    static constexpr int MP = WAVES_M * TM * 32;  // pixels per work-group
+    static constexpr int NC = WAVES_N * TN * 32;  // output channels per work-group
+    static constexpr int AS = KC + 8;             // LDS halves per staged pixel
+    static constexpr int R = TAPS == 9 ? 3 : 1;   // staged input rows
+    static constexpr int P = TAPS == 9 ? MP + 2 : MP;  // staged pixels per row
+    static constexpr int BS = TAPS * KC + 8;      // LDS halves per staged output channel
+    static constexpr int G = KC / 8;              // 16-byte groups per pixel / tap
+    static constexpr int NA = (R * P * G + 255) / 256;     // A groups per thread
+    static constexpr int NB = (NC * TAPS * G + 255) / 256;  // B groups per thread
+    static constexpr size_t lds_bytes = 2 * ((size_t)R * P * AS + (size_t)NC * BS);
+};
+
+// Staging of one channel chunk: global -> registers (conv_stage_load) and
+// registers -> LDS (conv_stage_store); thread tid owns groups tid + 256 k.
+// 16-byte groups travel as 4 x u32 vectors (kept in registers).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ __forceinline__ void conv_stage_load(u32x4 (&ra)[T::NA], u32x4 (&rb)[T::NB],
+                                                const uint16_t* __restrict__ x, int ldx, int cx0,
+                                                const uint16_t* __restrict__ w, int c0, int tid,
+                                                int row, int img, int x0, int co_base, int H,
+                                                int W, int cin, int cout)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int G = T::G, P = T::P, R = T::R, TAPS = T::TAPS, NC = T::NC;
+    constexpr int HALO = TAPS == 9 ? 1 : 0;
+#pragma unroll
+    for (int k = 0; k < T::NA; ++k) {
+        const int e = tid + 256 * k;
+        const int g = e % G, q = e / G, pp = q % P, r = q / P;
+        const int yy = row + r - HALO, xx = x0 + pp - HALO, c = c0 + 8 * g;
+        const bool ok = e < R * P * G && yy >= 0 && yy < H && xx >= 0 && xx < W && c < cin;
+        ra[k] = ok ? *reinterpret_cast<const u32x4*>(x + ((size_t)(img * H + yy) * W + xx) * ldx +
+                                                      cx0 + c)
+                   : u32x4{0u, 0u, 0u, 0u};
+    }
+#pragma unroll
+    for (int k = 0; k < T::NB; ++k) {
+        const int e = tid + 256 * k;
+        const int g = e % G, q = e / G, t = q % TAPS, co = q / TAPS;
+        const int cg = co_base + co, c = c0 + 8 * g;
+        const bool ok = e < NC * TAPS * G && cg < cout && c < cin;
+        rb[k] = ok ? *reinterpret_cast<const u32x4*>(w + ((size_t)cg * TAPS + t) * cin + c)
+                   : u32x4{0u, 0u, 0u, 0u};
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void conv_stage_store(const u32x4 (&ra)[T::NA],
+                                                 const u32x4 (&rb)[T::NB], uint16_t* As,
+                                                 uint16_t* Bs, int tid)
+{
+    constexpr int G = T::G, P = T::P, R = T::R, TAPS = T::TAPS, NC = T::NC;
+#pragma unroll
+    for (int k = 0; k < T::NA; ++k) {
+        const int e = tid + 256 * k;
+        if (e < R * P * G)
+            *reinterpret_cast<u32x4*>(As + (size_t)(e / G) * T::AS + 8 * (e % G)) = ra[k];
+    }
+#pragma unroll
+    for (int k = 0; k < T::NB; ++k) {
+        const int e = tid + 256 * k;
+        if (e < NC * TAPS * G) {
+            const int g = e % G, q = e / G, t = q % TAPS, co = q / TAPS;
+            *reinterpret_cast<u32x4*>(Bs + (size_t)co * T::BS + t * T::KC + 8 * g) = rb[k];
+        }
+    }
+}
+
+// The next chunk's input and weight groups are loaded into registers while
+// the current chunk computes, then written to LDS after a barrier.
+template <int WAVES_M, int TM, int TN, int KC, int TAPS>
+__global__ __launch_bounds__(256) void conv_tile_kernel(
+    const uint16_t* __restrict__ x, int ldx, int cx0, const uint16_t* __restrict__ w,
+    const float* __restrict__ bias, void* __restrict__ y, int ldy, int cy0, int H, int W,
+    int cin, int cout, int relu, int out_f32)
+{
+    using T = ConvTile<WAVES_M, TM, TN, KC, TAPS>;
+    constexpr int MP = T::MP, NC = T::NC, AS = T::AS, R = T::R, P = T::P, BS = T::BS, G = T::G;
+    constexpr int NA = T::NA, NB = T::NB, HALO = TAPS == 9 ? 1 : 0;
+    extern __shared__ __attribute__((aligned(16))) uint16_t lds[];
+    uint16_t* As = lds;                  // [R][P][AS]
+    uint16_t* Bs = lds + (size_t)R * P * AS;  // [NC][BS]
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, l32 = lane & 31;
-    const int tiles_x = (W + 63) / 64;
-    const int co_blocks = (cout + 63) / 64;
-    // block -> (image, row, 64-pixel segment, 64-channel block)
+    const int segs = (W + MP - 1) / MP, co_blocks = (cout + NC - 1) / NC;
     int b = blockIdx.x;
     const int cob = b % co_blocks;
     b /= co_blocks;
-    const int seg = b % tiles_x;
-    b /= tiles_x;
-    const int row = b % H;
-    const int img = b / H;
-    const int wm = wave & 1, wn = wave >> 1;
-    const int xbase = seg * 64 + wm * 32;   // first pixel x of this wave tile
-    const int co0 = cob * 64 + wn * 32;
-    if (co0 >= cout || xbase >= W) return;  // wave-uniform
-    const int px = xbase + l32;             // A row of this lane
-    const int co = co0 + l32;               // B column of this lane
+    const int seg = b % segs;
+    b /= segs;
+    const int row = b % H, img = b / H;
+    const int x0 = seg * MP, co_base = cob * NC;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
 
-    floatx16 acc;
+    floatx16 acc[TM][TN];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-    const int kpt = cin / 16;  // k-steps per tap
-    for (int t = 0; t < taps; ++t) {
-        const int dy = taps == 9 ? t / 3 - 1 : 0, dx = taps == 9 ? t % 3 - 1 : 0;
-        const int yy = row + dy, xx = px + dx;
-        const bool inb = px < W && yy >= 0 && yy < H && xx >= 0 && xx < W;
-        const uint16_t* xa =
-            x + ((size_t)(img * H + (inb ? yy : 0)) * W + (inb ? xx : 0)) * ldx + cx0 + 8 * h;
-        const uint16_t* wb = w + ((size_t)co * taps + t) * cin + 8 * h;
-        for (int kb = 0; kb < kpt; ++kb) {
-            U16x8 a, bb;
-            if (inb) a = *reinterpret_cast<const U16x8*>(xa + kb * 16);
-            else {
+    u32x4 ra[NA], rb[NB];
+    conv_stage_load<T>(ra, rb, x, ldx, cx0, w, 0, tid, row, img, x0, co_base, H, W, cin, cout);
+    for (int c0 = 0; c0 < cin; c0 += KC) {
+        __syncthreads();  // the previous chunk is no longer read
+        conv_stage_store<T>(ra, rb, As, Bs, tid);
+        __syncthreads();
+        if (c0 + KC < cin)  // in flight during this chunk's MFMAs
+            conv_stage_load<T>(ra, rb, x, ldx, cx0, w, c0 + KC, tid, row, img, x0, co_base, H, W,
+                               cin, cout);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) a.v[j] = 0;
+        for (int t = 0; t < TAPS; ++t) {
+            const int dy = TAPS == 9 ? t / 3 : 0, dx = TAPS == 9 ? t % 3 : 0;
+            const uint16_t* arow = As + (size_t)dy * P * AS;
+#pragma unroll
+            for (int k = 0; k < KC; k += 16) {
+                U16x8 a[TM], bb[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) {
+                    const int pl = (wm * TM + i) * 32 + l32 + dx;  // staged pixel index
+                    a[i] = *reinterpret_cast<const U16x8*>(arow + (size_t)pl * AS + k + 8 * h);
+                }
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const int cl = (wn * TN + j) * 32 + l32;
+                    bb[j] = *reinterpret_cast<const U16x8*>(Bs + (size_t)cl * BS + t * KC + k +
+                                                            8 * h);
+                }
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            as_bf16x8(a[i]), as_bf16x8(bb[j]), acc[i][j], 0, 0, 0);
             }
-            bb = *reinterpret_cast<const U16x8*>(wb + kb * 16);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), as_bf16x8(bb), acc, 0, 0, 0);
         }
     }
-    // epilogue: lane holds column co, rows m = 8(r/4) + 4h + r%4
-    const float bv = bias ? bias[co] : 0.f;
+    // epilogue: lane holds column co, rows m = 8(r/4) + 4h + r%4 of each tile
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int m = 8 * (r >> 2) + 4 * h + (r & 3);
-        const int xo = xbase + m;
-        if (xo >= W) continue;
-        float v = acc[r] + bv;
-        if (relu) v = v > 0.f ? v : 0.f;
-        const size_t pix = (size_t)(img * H + row) * W + xo;
-        if (out_f32)
-            reinterpret_cast<float*>(y)[pix * ldy + cy0 + co] = v;
-        else
-            reinterpret_cast<uint16_t*>(y)[pix * ldy + cy0 + co] = f2bf(v);
+    for (int j = 0; j < TN; ++j) {
+        const int co = co_base + (wn * TN + j) * 32 + l32;
+        if (co >= cout) continue;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = 8 * (r >> 2) + 4 * h + (r & 3);
+                const int xo = x0 + (wm * TM + i) * 32 + m;
+                if (xo >= W) continue;
+                float v = acc[i][j][r] + bv;
+                if (relu) v = v > 0.f ? v : 0.f;
+                const size_t pix = (size_t)(img * H + row) * W + xo;
+                if (out_f32)
+                    reinterpret_cast<float*>(y)[pix * ldy + cy0 + co] = v;
+                else
+                    reinterpret_cast<uint16_t*>(y)[pix * ldy + cy0 + co] = f2bf(v);
+            }
+        }
     }
 }
 
@@ -514,6 +622,25 @@ static bool vec8(int c, int ld, int c0, const void* base)
     return c % 8 == 0 && ld % 8 == 0 && c0 % 8 == 0 && ((uintptr_t)base & 15) == 0;
 }
 
+template <int WAVES_M, int TM, int TN, int KC>
+static void conv_tile_launch(const uint16_t* x, int ldx, int cx0, const uint16_t* w,
+                             const float* bias, void* y, int ldy, int cy0, int n, int H, int W,
+                             int cin, int cout, int taps, int relu, int out_f32, hipStream_t st)
+{
+    using T9 = ConvTile<WAVES_M, TM, TN, KC, 9>;
+    using T1 = ConvTile<WAVES_M, TM, TN, KC, 1>;
+    const long long blocks =
+        (long long)n * H * ((W + T9::MP - 1) / T9::MP) * ((cout + T9::NC - 1) / T9::NC);
+    if (taps == 9)
+        hipLaunchKernelGGL((conv_tile_kernel<WAVES_M, TM, TN, KC, 9>), dim3((unsigned)blocks),
+                           dim3(256), T9::lds_bytes, st, x, ldx, cx0, w, bias, y, ldy, cy0, H, W,
+                           cin, cout, relu, out_f32);
+    else
+        hipLaunchKernelGGL((conv_tile_kernel<WAVES_M, TM, TN, KC, 1>), dim3((unsigned)blocks),
+                           dim3(256), T1::lds_bytes, st, x, ldx, cx0, w, bias, y, ldy, cy0, H, W,
+                           cin, cout, relu, out_f32);
+}
+
 extern "C" {
 
 int gr_cnn_conv_fwd(const void* x, int ldx, int cx0, const void* w, const float* bias, void* y,
@@ -525,13 +652,30 @@ int gr_cnn_conv_fwd(const void* x, int ldx, int cx0, const void* w, const float*
         set_error("cnn_conv_fwd: unsupported shape (cin %% 16, cout %% 32, 8-aligned ld)");
         return GR_ERR_INVALID;
     }
-    const int tiles_x = (W + 63) / 64;
-    const int co_blocks = (cout + 63) / 64;
-    const long long blocks = (long long)n * H * tiles_x * co_blocks;
-    hipLaunchKernelGGL(conv_fwd_kernel, dim3((unsigned)blocks), dim3(256), 0, cnn_stream(stream),
-                       (const uint16_t*)x, ldx, cx0, (const uint16_t*)w, bias, y, ldy, cy0, n, H,
-                       W, cin, cout, taps, relu, out_f32);
-    GR_LAUNCH_CHECK("conv_fwd_kernel");
+    // tile shape by row width and output channels: wide rows take 128-pixel
+    // tiles, 64- and 32-pixel rows take tiles with more output channels
+    const hipStream_t st = cnn_stream(stream);
+    const uint16_t* xp = (const uint16_t*)x;
+    const uint16_t* wp = (const uint16_t*)w;
+    if (W <= 32 && cout % 256 == 0)
+        conv_tile_launch<1, 1, 2, 16>(xp, ldx, cx0, wp, bias, y, ldy, cy0, n, H, W, cin, cout,
+                                      taps, relu, out_f32, st);
+    else if (W <= 32 && cout % 128 == 0)
+        conv_tile_launch<1, 1, 1, 16>(xp, ldx, cx0, wp, bias, y, ldy, cy0, n, H, W, cin, cout,
+                                      taps, relu, out_f32, st);
+    else if (W <= 64 && cout % 128 == 0)
+        conv_tile_launch<2, 1, 2, 16>(xp, ldx, cx0, wp, bias, y, ldy, cy0, n, H, W, cin, cout,
+                                      taps, relu, out_f32, st);
+    else if (W <= 64 && cout % 64 == 0)
+        conv_tile_launch<2, 1, 1, 32>(xp, ldx, cx0, wp, bias, y, ldy, cy0, n, H, W, cin, cout,
+                                      taps, relu, out_f32, st);
+    else if (cout % 64 == 0)
+        conv_tile_launch<2, 2, 1, 32>(xp, ldx, cx0, wp, bias, y, ldy, cy0, n, H, W, cin, cout,
+                                      taps, relu, out_f32, st);
+    else
+        conv_tile_launch<4, 1, 1, 32>(xp, ldx, cx0, wp, bias, y, ldy, cy0, n, H, W, cin, cout,
+                                      taps, relu, out_f32, st);
+    GR_LAUNCH_CHECK("conv_tile_kernel");
     return GR_OK;
 }
 

@@ -48,6 +48,11 @@ def rel(a, b):
     (9, 16, 32, 64, True),    # K = 16 (one k-step per tap), relu
     (1, 32, 32, 72, False),   # 1x1 head shape
     (9, 48, 96, 24, False),   # co block with only one of two wave columns live
+    # the tile shapes picked by row width and output channels
+    (9, 32, 128, 70, False),  # 128-pixel x 64-channel tiles, partial segment
+    (9, 64, 128, 50, True),   # 64 x 128 tiles, 16-channel chunks
+    (9, 16, 256, 20, False),  # 32 x 256 tiles
+    (1, 64, 128, 30, False),  # 32 x 128 tiles, 1x1
 ])
 def test_conv_forward_f32(taps, cin, cout, W, relu):
     rng = np.random.default_rng(10 + taps + cin)
