@@ -230,84 +230,105 @@ __global__ __launch_bounds__(256) void conv_tile_kernel(
 // ---------------------------------------------------------------------------
 // Weight gradient: dW[co][tap][ci] += sum over pixels of
 // dy[p][co] * x[p shifted by tap][ci].  M = co (32 per wave), N = ci (32 per
-// wave), K = pixels, staged through LDS transposed so both MFMA operands are
-// contiguous 16-byte reads.  A work-group covers 64 co x 64 ci of one tap
-// over a chunk of rows; partial sums are added to the FP32 accumulator with
-// atomics (split-K over the chunks).
+// wave), K = pixels.  A work-group covers 64 co x 64 ci and one row of taps
+// (the three taps of one vertical offset, or the single 1x1 tap) over a chunk
+// of rows; per 64-pixel step it stages dy as [co][px] and the input row as
+// [shift][ci][px] (one copy per horizontal shift, so every MFMA operand is
+// an aligned 16-byte LDS read), then runs 4 k steps x the row's taps.
+// Partial sums go to the FP32 accumulator with atomics (split-K over the
+// chunks).
 // ---------------------------------------------------------------------------
-constexpr int WG_PX = 32;  // pixels per k-step pair (two MFMAs of K = 16)
+constexpr int WG_PX = 64;            // pixels per staged step (4 MFMA k steps)
+constexpr int WG_LD = WG_PX + 8;     // LDS row stride (halves): conflict-free fragment reads
 
+template <int TAPS>
 __global__ __launch_bounds__(256) void conv_wgrad_kernel(
     const uint16_t* __restrict__ x, int ldx, int cx0, const uint16_t* __restrict__ dy, int ldd,
-    int cd0, float* __restrict__ dw, int n, int H, int W, int cin, int cout, int taps,
-    int rows_per_chunk)
+    int cd0, float* __restrict__ dw, int n, int H, int W, int cin, int cout, int rows_per_chunk)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t sdy[64][WG_PX + 8];  // [co][px]
-    __shared__ __attribute__((aligned(16))) uint16_t sx[64][WG_PX + 8];   // [ci][px]
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NS = TAPS == 9 ? 3 : 1;  // horizontal shifts = taps per work-group
+    __shared__ __attribute__((aligned(16))) uint16_t sdy[64][WG_LD];      // [co][px]
+    __shared__ __attribute__((aligned(16))) uint16_t sx[NS][64][WG_LD];   // [shift][ci][px]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, l32 = lane & 31;
     const int co_blocks = (cout + 63) / 64, ci_blocks = (cin + 63) / 64;
     int b = blockIdx.x;
     const int cob = b % co_blocks; b /= co_blocks;
     const int cib = b % ci_blocks; b /= ci_blocks;
-    const int t = b % taps; b /= taps;
-    const int chunk = b;  // chunk of rows over all images
+    const int trow = b % NS; b /= NS;  // vertical offset index (3x3) or 0 (1x1)
+    const int chunk = b;               // chunk of rows over all images
     const int total_rows = n * H;
     const int r0 = chunk * rows_per_chunk;
     const int r1 = min(total_rows, r0 + rows_per_chunk);
-    const int dyo = taps == 9 ? t / 3 - 1 : 0, dxo = taps == 9 ? t % 3 - 1 : 0;
+    const int dyo = TAPS == 9 ? trow - 1 : 0;
     const int wm = wave & 1, wn = wave >> 1;
 
-    floatx16 acc;
+    floatx16 acc[NS];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    for (int d = 0; d < NS; ++d)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[d][r] = 0.f;
 
-    const int tid = threadIdx.x;
     for (int gr_ = r0; gr_ < r1; ++gr_) {
         const int img = gr_ / H, row = gr_ - img * H;
         const int yy = row + dyo;
         for (int x0 = 0; x0 < W; x0 += WG_PX) {
             __syncthreads();
-            // stage 32 pixels x 64 channels of dy and of shifted x (8 channels
-            // per thread: 32 px * 8 groups = 256 threads)
-            {
-                const int p = tid & 31, g = tid >> 5;  // g = 0..7 channel group
-                const int xo = x0 + p;
-                const int c = g * 8;
-                U16x8 vd, vx;
-                const int co = cob * 64 + c, ci = cib * 64 + c;
-                if (co < cout && xo < W)
-                    vd = *reinterpret_cast<const U16x8*>(
-                        dy + ((size_t)(img * H + row) * W + xo) * ldd + cd0 + co);
-                else
-                    for (int j = 0; j < 8; ++j) vd.v[j] = 0;
-                const int xs = xo + dxo;
-                if (ci < cin && xo < W && yy >= 0 && yy < H && xs >= 0 && xs < W)
-                    vx = *reinterpret_cast<const U16x8*>(
-                        x + ((size_t)(img * H + yy) * W + xs) * ldx + cx0 + ci);
-                else
-                    for (int j = 0; j < 8; ++j) vx.v[j] = 0;
+            // dy: 64 px x 8 channel groups (two per thread)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    sdy[c + j][p] = vd.v[j];
-                    sx[c + j][p] = vx.v[j];
+            for (int k = 0; k < 2; ++k) {
+                const int e = tid + 256 * k, p = e & 63, g = e >> 6;
+                const int xo = x0 + p, co = cob * 64 + 8 * g;
+                U16x8 v = {{0, 0, 0, 0, 0, 0, 0, 0}};
+                if (co < cout && xo < W)
+                    v = *reinterpret_cast<const U16x8*>(
+                        dy + ((size_t)(img * H + row) * W + xo) * ldd + cd0 + co);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) sdy[8 * g + j][p] = v.v[j];
+            }
+            // x: pixels x0 - 1 .. x0 + 64 of row yy (3x3) or x0 .. x0 + 63 (1x1),
+            // written once per horizontal shift they serve
+            constexpr int QP = TAPS == 9 ? WG_PX + 2 : WG_PX;
+            for (int e = tid; e < QP * 8; e += 256) {
+                const int q = e % QP, g = e / QP;
+                const int xs = x0 + q - (TAPS == 9 ? 1 : 0), ci = cib * 64 + 8 * g;
+                U16x8 v = {{0, 0, 0, 0, 0, 0, 0, 0}};
+                if (ci < cin && yy >= 0 && yy < H && xs >= 0 && xs < W)
+                    v = *reinterpret_cast<const U16x8*>(
+                        x + ((size_t)(img * H + yy) * W + xs) * ldx + cx0 + ci);
+#pragma unroll
+                for (int d = 0; d < NS; ++d) {
+                    const int p = q - d;  // shift d reads pixel x0 + p + d - 1
+                    if (p >= 0 && p < WG_PX) {
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) sx[d][8 * g + j][p] = v.v[j];
+                    }
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int ks = 0; ks < WG_PX / 16; ++ks) {
-                U16x8 a = *reinterpret_cast<const U16x8*>(&sdy[wm * 32 + l32][ks * 16 + 8 * h]);
-                U16x8 bb = *reinterpret_cast<const U16x8*>(&sx[wn * 32 + l32][ks * 16 + 8 * h]);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), as_bf16x8(bb), acc, 0,
-                                                              0, 0);
+                const U16x8 a =
+                    *reinterpret_cast<const U16x8*>(&sdy[wm * 32 + l32][ks * 16 + 8 * h]);
+#pragma unroll
+                for (int d = 0; d < NS; ++d) {
+                    const U16x8 bb =
+                        *reinterpret_cast<const U16x8*>(&sx[d][wn * 32 + l32][ks * 16 + 8 * h]);
+                    acc[d] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(a), as_bf16x8(bb),
+                                                                     acc[d], 0, 0, 0);
+                }
             }
         }
     }
     const int ci = cib * 64 + wn * 32 + l32;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int co = cob * 64 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        if (co < cout && ci < cin) atomicAdd(&dw[((size_t)co * taps + t) * cin + ci], acc[r]);
+    for (int d = 0; d < NS; ++d) {
+        const int t = TAPS == 9 ? 3 * trow + d : 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int co = cob * 64 + wm * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            if (co < cout && ci < cin) atomicAdd(&dw[((size_t)co * TAPS + t) * cin + ci], acc[d][r]);
+        }
     }
 }
 
@@ -328,6 +349,42 @@ __global__ __launch_bounds__(256) void bias_grad_kernel(const uint16_t* __restri
     if (lane == 0) {
         for (int l = 1; l < lanes; ++l) acc += red[l * cout + co];
         atomicAdd(&db[co], acc);
+    }
+}
+
+// The same sum with 16-byte reads: a thread owns 8 consecutive channels of
+// one pixel lane (cout / 8 groups, 256 * 8 / cout pixel lanes per block).
+__global__ __launch_bounds__(256) void bias_grad8_kernel(const uint16_t* __restrict__ dy,
+                                                         int ldd, int cd0, size_t npix, int cout,
+                                                         float* __restrict__ db)
+{
+    __shared__ float red[8][256];
+    const int groups = cout / 8, lanes = 256 / groups;
+    const int g = threadIdx.x % groups, lane = threadIdx.x / groups;
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const size_t stride = (size_t)gridDim.x * lanes;
+    // four independent loads in flight per thread
+    for (size_t p = (size_t)blockIdx.x * lanes + lane; p < npix; p += 4 * stride) {
+        U16x8 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t q = p + u * stride;
+            v[u] = q < npix ? *reinterpret_cast<const U16x8*>(dy + q * ldd + cd0 + 8 * g)
+                            : U16x8{{0, 0, 0, 0, 0, 0, 0, 0}};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += bf2f(v[u].v[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) red[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    if (threadIdx.x < cout) {  // one thread per channel sums the pixel lanes
+        const int c = threadIdx.x, gg = c / 8, j = c % 8;
+        float s = 0.f;
+        for (int l = 0; l < lanes; ++l) s += red[j][l * groups + gg];
+        atomicAdd(&db[c], s);
     }
 }
 
@@ -690,11 +747,16 @@ int gr_cnn_conv_wgrad(const void* x, int ldx, int cx0, const void* dy, int ldd, 
         return GR_ERR_INVALID;
     }
     const int chunks = (n * H + rows_per_chunk - 1) / rows_per_chunk;
-    const long long blocks =
-        (long long)((cout + 63) / 64) * ((cin + 63) / 64) * taps * chunks;
-    hipLaunchKernelGGL(conv_wgrad_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                       cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (const uint16_t*)dy, ldd,
-                       cd0, dw, n, H, W, cin, cout, taps, rows_per_chunk);
+    const long long blocks = (long long)((cout + 63) / 64) * ((cin + 63) / 64) *
+                             (taps == 9 ? 3 : 1) * chunks;
+    if (taps == 9)
+        hipLaunchKernelGGL(conv_wgrad_kernel<9>, dim3((unsigned)blocks), dim3(256), 0,
+                           cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (const uint16_t*)dy,
+                           ldd, cd0, dw, n, H, W, cin, cout, rows_per_chunk);
+    else
+        hipLaunchKernelGGL(conv_wgrad_kernel<1>, dim3((unsigned)blocks), dim3(256), 0,
+                           cnn_stream(stream), (const uint16_t*)x, ldx, cx0, (const uint16_t*)dy,
+                           ldd, cd0, dw, n, H, W, cin, cout, rows_per_chunk);
     GR_LAUNCH_CHECK("conv_wgrad_kernel");
     return GR_OK;
 }
@@ -706,6 +768,16 @@ int gr_cnn_bias_grad(const void* dy, int ldd, int cd0, size_t npix, int cout, fl
         !slice_ok(ldd, cd0, cout)) {
         set_error("cnn_bias_grad: invalid arguments (cout must divide 256)");
         return GR_ERR_INVALID;
+    }
+    if (cout % 8 == 0 && vec8(cout, ldd, cd0, dy)) {
+        // one block per CU: every block ends in cout atomics on the same cout
+        // addresses, so fewer, longer blocks
+        const size_t lanes = 256 / (cout / 8);
+        const unsigned blocks = (unsigned)std::min<size_t>(256, (npix + lanes - 1) / lanes);
+        hipLaunchKernelGGL(bias_grad8_kernel, dim3(blocks), dim3(256), 0, cnn_stream(stream),
+                           (const uint16_t*)dy, ldd, cd0, npix, cout, db);
+        GR_LAUNCH_CHECK("bias_grad8_kernel");
+        return GR_OK;
     }
     const size_t lanes = 256 / cout;
     const unsigned blocks = (unsigned)std::min<size_t>(2048, (npix + lanes - 1) / lanes);

@@ -265,7 +265,8 @@ class UNet:
                 _lib.check(lib.gr_cnn_relu_bwd(dy.ptr, dy.ld, dy.c0, yout.ptr, yout.ld, yout.c0,
                                                dy.npix(), l.cout, st), "relu_bwd")
             rows = n * xin.h
-            blocks = ((l.cout + 63) // 64) * ((l.cin + 63) // 64) * l.taps
+            # work-groups per chunk: 64 x 64 channel tiles x rows of taps
+            blocks = ((l.cout + 63) // 64) * ((l.cin + 63) // 64) * (3 if l.taps == 9 else 1)
             chunks = max(1, min(rows, 1024 // max(blocks, 1)))
             rpc = max(1, (rows + chunks - 1) // chunks)
             _lib.check(lib.gr_cnn_conv_wgrad(xin.ptr, xin.ld, xin.c0, dy.ptr, dy.ld, dy.c0,
