@@ -142,7 +142,7 @@ int adj_segw_of(int tile_y);
 int adj_achunk_of(int na, int segw, int K);
 size_t fwd_lds_bytes(const ProbDev& P, int K);
 size_t adj_lds_bytes(const ProbDev& P, int K);
-int slice_group(int nslices);  // K = 4, 2 or 1 slices sharing geometry
+int slice_group(int nslices);  // K = 4 (3 or more slices), 2 or 1 slices sharing geometry
 
 // ---- launchers (single problem: descriptor by value) ---------------------
 int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipStream_t st);

@@ -529,7 +529,9 @@ size_t adj_lds_bytes(const ProbDev& P, int K)
                                    sizeof(int));
 }
 
-int slice_group(int nslices) { return nslices >= 4 ? 4 : (nslices >= 2 ? 2 : 1); }
+// three slices take a group of four with one padded slot: one pass instead of
+// a pair and a single
+int slice_group(int nslices) { return nslices >= 3 ? 4 : (nslices == 2 ? 2 : 1); }
 
 // --------------------------------------------------------------------------
 // single-problem launchers
