@@ -267,7 +267,7 @@ __device__ __forceinline__ void primal_update_r(
 {
     constexpr bool NU = (PART & PRIMAL_U) != 0, NP = (PART & PRIMAL_P) != 0;
     float un, ub, pn[3], pb[3];
-    primal_point<DIM, TGV, NU, NP>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
+    primal_point<DIM, TGV, NU, NP, true>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
                                    z, un, ub, pn, pb);
     if constexpr (NU) {
         u[i] = un;

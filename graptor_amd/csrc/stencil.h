@@ -17,7 +17,25 @@
 
 namespace gr {
 
-template <int DIM, bool TGV, bool NEED_U, bool NEED_P>
+// Values of the neighbouring lanes of a wave (wave-wide DPP shifts by one
+// lane); the lane at the wave's edge receives 0 and must take its value from
+// memory.
+__device__ __forceinline__ float lane_prev(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138 /* wave_shr:1 */,
+                                                      0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130 /* wave_shl:1 */,
+                                                      0xf, 0xf, false));
+}
+
+// LANEX: the 64 lanes of the calling wave hold 64 consecutive x of one row
+// (lane = x - x0, every lane below the last active one active), so the x - 1
+// and x + 1 neighbours of v and w come from the neighbouring lanes and only
+// the wave's edge lanes read them from memory.
+template <int DIM, bool TGV, bool NEED_U, bool NEED_P, bool LANEX = false>
 __device__ __forceinline__ void primal_point(
     const float* __restrict__ u, const float* __restrict__ p, const float* __restrict__ v,
     const float* __restrict__ w, float ktl, int nx, int ny, int nz, size_t cs, float hinv,
@@ -28,9 +46,16 @@ __device__ __forceinline__ void primal_point(
     float vc[3];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
+    const int lane = (int)(threadIdx.x & 63);
     if constexpr (NEED_U) {
         float d0 = (x == nx - 1) ? 0.f : vc[0];
-        if (x > 0) d0 -= v[i - 1];
+        if constexpr (LANEX) {
+            float vm = lane_prev(vc[0]);
+            if (lane == 0 && x > 0) vm = v[i - 1];
+            if (x > 0) d0 -= vm;
+        } else {
+            if (x > 0) d0 -= v[i - 1];
+        }
         float d1 = (y == ny - 1) ? 0.f : vc[1];
         if (y > 0) d1 -= v[i + cs - nx];
         float dv = d0 + d1;
@@ -53,7 +78,13 @@ __device__ __forceinline__ void primal_point(
         if constexpr (DIM == 2) {
             // x-part uses (w_xx, w_xy), y-part (w_xy, w_yy)
             float x0 = (x == 0) ? 0.f : -wc[0], x1 = (x == 0) ? 0.f : -wc[2];
-            if (x < nx - 1) { x0 += w[i + 1]; x1 += w[i + 2 * cs + 1]; }
+            if constexpr (LANEX) {
+                float n0 = lane_next(wc[0]), n1 = lane_next(wc[2]);
+                if (lane == 63 && x < nx - 1) { n0 = w[i + 1]; n1 = w[i + 2 * cs + 1]; }
+                if (x < nx - 1) { x0 += n0; x1 += n1; }
+            } else {
+                if (x < nx - 1) { x0 += w[i + 1]; x1 += w[i + 2 * cs + 1]; }
+            }
             float y0 = (y == 0) ? 0.f : -wc[2], y1 = (y == 0) ? 0.f : -wc[1];
             if (y < ny - 1) { y0 += w[i + 2 * cs + nx]; y1 += w[i + cs + nx]; }
             q[0] = x0 + y0;
@@ -65,7 +96,13 @@ __device__ __forceinline__ void primal_point(
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 float xs = (x == 0) ? 0.f : -wc[xi[k]];
-                if (x < nx - 1) xs += w[i + xi[k] * cs + 1];
+                if constexpr (LANEX) {
+                    float nb = lane_next(wc[xi[k]]);
+                    if (lane == 63 && x < nx - 1) nb = w[i + xi[k] * cs + 1];
+                    if (x < nx - 1) xs += nb;
+                } else {
+                    if (x < nx - 1) xs += w[i + xi[k] * cs + 1];
+                }
                 float ys = (y == 0) ? 0.f : -wc[yi[k]];
                 if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
                 float zs = (z == 0) ? 0.f : -wc[zi[k]];

@@ -108,6 +108,8 @@ __device__ __forceinline__ void jacobi3(const T (&A)[6], T (&ev)[3], T (&V)[3][3
 // iteration k and the dual step of iteration k + 1.
 template <int DIM, bool TGV>
 struct MemSrc {
+    // callers run 64 consecutive x per wave: x +- 1 neighbours come from lanes
+    static constexpr bool lanex = true;
     const float* __restrict__ ubar;
     const float* __restrict__ pbar;
     size_t cs;
@@ -130,6 +132,7 @@ struct MemSrc {
 
 template <int DIM, bool TGV>
 struct FusedSrc {
+    static constexpr bool lanex = false;
     // previous iterate (read only in the kernel)
     const float* __restrict__ u;
     const float* __restrict__ p;
@@ -202,7 +205,13 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
         float p[3] = {0.f, 0.f, 0.f};
         src.own(c, ic, x, y, z, u0, p);
         float g[3];
-        g[0] = (x < nx - 1) ? src.ubn(c, ic + 1, x + 1, y, z) - u0 : 0.f;
+        if constexpr (Src::lanex) {
+            float un = lane_next(u0);
+            if ((threadIdx.x & 63) == 63 && x < nx - 1) un = src.ubn(c, ic + 1, x + 1, y, z);
+            g[0] = (x < nx - 1) ? un - u0 : 0.f;
+        } else {
+            g[0] = (x < nx - 1) ? src.ubn(c, ic + 1, x + 1, y, z) - u0 : 0.f;
+        }
         g[1] = (y < ny - 1) ? src.ubn(c, ic + nx, x, y + 1, z) - u0 : 0.f;
         if constexpr (DIM == 3)
             g[2] = (z < nz - 1) ? (src.ubn(c, ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
@@ -214,7 +223,15 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
         if constexpr (TGV) {
             float ax[3] = {0.f, 0.f, 0.f}, ay[3] = {0.f, 0.f, 0.f}, az[3] = {0.f, 0.f, 0.f};
             float q[3];
-            if (x > 0) {
+            if constexpr (Src::lanex) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) q[k] = lane_prev(p[k]);
+                if ((threadIdx.x & 63) == 0 && x > 0) src.pbn(c, ic - 1, x - 1, y, z, q);
+                if (x > 0) {
+#pragma unroll
+                    for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
+                }
+            } else if (x > 0) {
                 src.pbn(c, ic - 1, x - 1, y, z, q);
 #pragma unroll
                 for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
@@ -580,6 +597,7 @@ struct FzPlane {
 
 template <int MAXC, bool TGV>
 struct FzSrc {
+    static constexpr bool lanex = false;
     const FzPlane<MAXC, TGV>* sh;
     float un[MAXC];       // ubar at (x, y, z + 1)
     float pp[MAXC][3];    // pbar at (x, y, z - 1)
