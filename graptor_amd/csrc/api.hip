@@ -227,24 +227,24 @@ int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_o
     // dual_k+1 waits for the u half of primal_k (it reads ubar_k+1); the
     // halves write disjoint arrays, the projections only read ubar and lam.
     Lane* lane = (!ms_out && !P.fuse_primal) ? lane_for(st) : nullptr;
+    auto hip_rc = [](hipError_t e, const char* where) {
+        return e == hipSuccess ? GR_OK : hip_status(e, where);
+    };
     for (; lane && it < plain && rc == GR_OK; ++it) {
-        GR_HIP(hipEventRecord(lane->fork, st));
-        GR_HIP(hipStreamWaitEvent(lane->side, lane->fork, 0));
-        rc = launch_dual(P, lane->side);
-        if (!rc) {
-            GR_HIP(hipEventRecord(lane->dual_done, lane->side));
-            rc = launch_primal_part(P, PRIMAL_P, lane->side);
-        }
+        rc = hip_rc(hipEventRecord(lane->fork, st), "fork");
+        if (!rc) rc = hip_rc(hipStreamWaitEvent(lane->side, lane->fork, 0), "fork");
+        if (!rc) rc = launch_dual(P, lane->side);
+        if (!rc) rc = hip_rc(hipEventRecord(lane->dual_done, lane->side), "dual event");
+        if (!rc) rc = launch_primal_part(P, PRIMAL_P, lane->side);
         if (!rc) rc = launch_fwd(P, gg->g, nsl, fwd_mode(it), st);
         if (!rc) rc = launch_adj(P, nsl, false, st);
-        if (!rc) {
-            GR_HIP(hipStreamWaitEvent(st, lane->dual_done, 0));
-            rc = launch_primal_part(P, PRIMAL_U, st);
-        }
+        if (!rc) rc = hip_rc(hipStreamWaitEvent(st, lane->dual_done, 0), "dual wait");
+        if (!rc) rc = launch_primal_part(P, PRIMAL_U, st);
     }
-    if (lane) {  // the caller's stream ends after everything of the side stream
-        GR_HIP(hipEventRecord(lane->join, lane->side));
-        GR_HIP(hipStreamWaitEvent(st, lane->join, 0));
+    if (lane) {  // the caller's stream ends after everything of the side stream, also on error
+        const int jr = hip_rc(hipEventRecord(lane->join, lane->side), "join");
+        const int jw = jr ? jr : hip_rc(hipStreamWaitEvent(st, lane->join, 0), "join");
+        if (!rc) rc = jw;
     }
     for (; it < plain && rc == GR_OK; ++it) {
         rc = t.run(&ms[0], [&] { return launch_dual(P, st); });
