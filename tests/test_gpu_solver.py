@@ -220,6 +220,52 @@ def test_two_stream_iteration_matches_one_stream(combo):
             assert torch.equal(a, b), (n, name)
 
 
+def test_two_streams_from_two_host_threads():
+    """Each host thread gets its own side stream: two problems iterated from
+    two threads on their own torch streams give the same bits as the same
+    problems iterated one after the other."""
+    import threading
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    _, f, ofs = _problem(3, C=3, N=48, na=20)
+    C, Nz, na, N = f.shape
+    ang = np.deg2rad(np.linspace(-70, 70, na, endpoint=False)) + np.pi / 2
+    L = O.normest_power(ofs, N, N, N)
+    g = RadonGeometry((N, N), ang, n_detectors=N)
+
+    def make(scale):
+        return TgvProblem(f * scale, g, (4, 1), [1.0, 0.3, 0.1], dim=3, L=L)
+
+    ref = [make(1.0), make(2.0)]
+    for p in ref:
+        p.iterate(25)
+    torch.cuda.synchronize()
+    thr = [make(1.0), make(2.0)]
+    torch.cuda.synchronize()  # set-up work of the default stream is done
+    errors = []
+
+    def run(p):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(5):
+                    p.iterate(5)
+            s.synchronize()
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=run, args=(p,)) for p in thr]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for a, b in zip(thr, ref):
+        for name in ("u", "p", "v", "w", "lam"):
+            assert torch.equal(getattr(a, name), getattr(b, name)), name
+
+
 @pytest.mark.parametrize("combo", COMBOS, ids=["-".join(map(str, c)) for c in COMBOS])
 def test_fused_primal_dual_kernel_matches_separate_kernels(combo):
     """The primal step of each iteration fused with the next dual step (two
