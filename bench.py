@@ -380,6 +380,69 @@ def unet_main(args):
     print(json.dumps(line), flush=True)
 
 
+def slab_main(args):
+    """One GPU's share of the 512x512x256x4, 90-tilt volume split into 8
+    z-slabs: a 512x512x32x4 slab with halo planes on both sides (a middle
+    rank), iterated with the halo-aware kernels, and the plane copies of its
+    two halo exchanges per iteration (what a rank would send to / receive
+    from its neighbours over xGMI)."""
+    import torch
+
+    from graptor_amd.phantoms import coupled_granule_problem
+    from graptor_amd.solver import TgvProblem
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N, nz, na, ranks = 512, 32, 90, 8
+    f, angles, phantom, geom = coupled_granule_problem(N=N, Nz=nz, na=na, device=dev)
+    del phantom
+    prob = TgvProblem(f, geom, ALPHA, MU, dim=3, coupling="frobenius", discrepancy="KL",
+                      normest_mode="power", device=dev)
+    C = prob.C
+    plane = (C, N, N)
+    halo = {"ubar_hi": torch.empty(plane, device=dev), "pbar_lo": torch.empty((3,) + plane, device=dev),
+            "v_lo": torch.empty(plane, device=dev), "w_hi": torch.empty((3,) + plane, device=dev)}
+    # neighbours' planes stand in as copies of the slab's own edge planes
+    def exchange():
+        halo["ubar_hi"].copy_(prob.ubar[:, -1])
+        halo["pbar_lo"].copy_(prob.pbar[:, :, 0])
+        halo["v_lo"].copy_(prob.v[2, :, 0])
+        for j, m in enumerate((4, 5, 2)):
+            halo["w_hi"][j].copy_(prob.w[m, :, -1])
+    exchange()
+    prob.state.halo_ubar_hi = halo["ubar_hi"].data_ptr()
+    prob.state.halo_pbar_lo = halo["pbar_lo"].data_ptr()
+    prob.state.halo_v_lo = halo["v_lo"].data_ptr()
+    prob.state.halo_w_hi = halo["w_hi"].data_ptr()
+    prob.iterate(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prob.iterate(args.steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        exchange()
+    e1.record()
+    e1.synchronize()
+    ex_ms = e0.elapsed_time(e1) / args.steps
+    halo_bytes = sum(t.numel() for t in halo.values()) * 4  # received per iteration
+    line = {
+        "metric": "voxel-iters/s (one z-slab of a slab-sharded coupled TGV-KL volume)",
+        "value": prob.voxels() * args.steps / wall, "unit": "voxel-iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded granule phantom, Poisson counts)",
+        "config": {"workload": "512x512x32 slab x 4 channels (1/8 of 512x512x256), 90 tilts, "
+                               "halo planes above and below", "ranks_modelled": ranks},
+        "halo": {"bytes_received_per_iteration": halo_bytes,
+                 "copy_ms_per_iteration": ex_ms,
+                 "xgmi_ms_per_iteration_at_150GBps_per_link": halo_bytes / 2 / 150e9 * 1e3},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,7 +454,7 @@ def main():
     ap.add_argument("--nz", type=int, default=64)
     ap.add_argument("--workload", default="coupled3d",
                     choices=["coupled3d", "large3d", "batched2d", "warmstart", "unet",
-                             "ellipse2d"])
+                             "ellipse2d", "slab"])
     ap.add_argument("--cap-iters", type=int, default=2000,
                     help="large3d: iteration cap of the run to a relative-change tolerance")
     ap.add_argument("--change-tol", type=float, default=1e-3)
@@ -412,8 +475,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(z-slab sharding is a design note); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU transport "
+              "(one rank's z-slab with halo planes: --workload slab); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched2d":
         return batched_main(args)
@@ -423,6 +487,8 @@ def main():
         return unet_main(args)
     if args.workload == "ellipse2d":
         return ellipse_main(args)
+    if args.workload == "slab":
+        return slab_main(args)
 
     import numpy as np
     import torch

@@ -61,7 +61,7 @@ class TgvParams(ctypes.Structure):
 class TgvState(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
                 ("u", "ubar", "p", "pbar", "v", "w", "lam", "ktl", "ku", "f", "mfac",
-                 "u2", "v2", "w2")]
+                 "u2", "v2", "w2", "halo_ubar_hi", "halo_pbar_lo", "halo_v_lo", "halo_w_hi")]
 
 
 _lib = None

@@ -94,6 +94,14 @@ bool valid_state(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_stat
     if (!g->g.fuse_primal && !s->ktl) return false;
     if (tgv && (!s->p || !s->pbar || !s->w)) return false;
     if (need_ku && !s->ku) return false;
+    // a halo side is all of its planes (the p / w ones only with TGV)
+    const bool lo = s->halo_v_lo != nullptr, hi = s->halo_ubar_hi != nullptr;
+    if ((lo || hi) && p->dim != 3) return false;
+    if (!lo && s->halo_pbar_lo) return false;
+    if (!hi && s->halo_w_hi) return false;
+    if (tgv && lo && !s->halo_pbar_lo) return false;
+    if (tgv && hi && !s->halo_w_hi) return false;
+    if ((lo || hi) && (g->g.fuse_primal || s->u2)) return false;
     return true;
 }
 
@@ -122,6 +130,8 @@ ProbDev make_prob(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
     P.fwd = g.fwd_dev;
     P.u = s->u; P.ubar = s->ubar; P.p = s->p; P.pbar = s->pbar; P.v = s->v; P.w = s->w;
     P.lam = s->lam; P.ktl = s->ktl; P.ku = s->ku; P.f = s->f; P.mfac = s->mfac;
+    P.halo_ubar_hi = s->halo_ubar_hi; P.halo_pbar_lo = s->halo_pbar_lo;
+    P.halo_v_lo = s->halo_v_lo; P.halo_w_hi = s->halo_w_hi;
     return P;
 }
 
@@ -795,11 +805,16 @@ int gr_plan_create(int nprob, const gr_geometry* const* geoms, const gr_tgv_para
         return GR_ERR_INVALID;
     }
     *out = nullptr;
-    for (int i = 0; i < nprob; ++i)
+    for (int i = 0; i < nprob; ++i) {
         if (!valid_params(geoms[i], &prms[i]) || !valid_state(geoms[i], &prms[i], &states[i], false)) {
             set_error("plan_create: problem %d has invalid parameters or state", i);
             return GR_ERR_INVALID;
         }
+        if (states[i].halo_ubar_hi || states[i].halo_v_lo) {
+            set_error("plan_create: problem %d has z halo planes (slabs run singly)", i);
+            return GR_ERR_UNSUPPORTED;
+        }
+    }
     gr_plan* pl = new (std::nothrow) gr_plan();
     if (!pl) {
         set_error("plan_create: host allocation failed");

@@ -63,6 +63,9 @@ struct ProbDev {
     // out (forward), lam = sinogram in, ktl = image out (backprojection)
     float *u, *ubar, *p, *pbar, *v, *w, *lam, *ktl, *ku;
     const float *f, *mfac;
+    // z halo of a slab (3D; NULL: boundary plane), planes [c][y][x]:
+    // ubar above, pbar below [k][c], v_z below, (w_xz, w_yz, w_zz) above [j][c]
+    const float *halo_ubar_hi, *halo_pbar_lo, *halo_v_lo, *halo_w_hi;
 };
 
 // Buffers of the fused primal + next-dual step: the previous state is read

@@ -258,17 +258,17 @@ __global__ __launch_bounds__(FWD_THREADS) void fwd_batch_kernel(const ProbDev* _
 //   p+ = p + tau (v + div2 w),               pbar = 2 p+ - p
 // i indexes u[c][z][y][x] (= slice s = z + nz c), z = s % nz.
 // --------------------------------------------------------------------------
-template <int DIM, bool TGV, int PART = PRIMAL_UP>
+template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false>
 __device__ __forceinline__ void primal_update_r(
     float* __restrict__ u, float* __restrict__ ubar, float* __restrict__ p,
     float* __restrict__ pbar, const float* __restrict__ v, const float* __restrict__ w, int nx,
     int ny, int nz, size_t cs, float hinv, float tau, float norming, size_t i, int x, int y, int z,
-    float ktl)
+    float ktl, const ProbDev* zh)
 {
     constexpr bool NU = (PART & PRIMAL_U) != 0, NP = (PART & PRIMAL_P) != 0;
     float un, ub, pn[3], pb[3];
-    primal_point<DIM, TGV, NU, NP, true>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
-                                   z, un, ub, pn, pb);
+    primal_point<DIM, TGV, NU, NP, true, ZH>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
+                                   z, un, ub, pn, pb, zh);
     if constexpr (NU) {
         u[i] = un;
         ubar[i] = ub;
@@ -282,18 +282,18 @@ __device__ __forceinline__ void primal_update_r(
     }
 }
 
-template <int DIM, bool TGV, int PART = PRIMAL_UP>
+template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false>
 __device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x, int y, int z,
                                               float ktl)
 {
     const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;  // component stride
-    primal_update_r<DIM, TGV, PART>(P.u, P.ubar, P.p, P.pbar, P.v, P.w, P.nx, P.ny, P.nz, cs, P.hinv,
-                              P.tau, P.norming, i, x, y, z, ktl);
+    primal_update_r<DIM, TGV, PART, ZH>(P.u, P.ubar, P.p, P.pbar, P.v, P.w, P.nx, P.ny, P.nz, cs, P.hinv,
+                              P.tau, P.norming, i, x, y, z, ktl, &P);
 }
 
 // Primal step of one voxel-channel per thread with K^T lam read from memory
 // (the unfused form: high occupancy for the memory-bound update).
-template <int DIM, bool TGV, int PART = PRIMAL_UP>
+template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false>
 __device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
 {
     const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
@@ -302,19 +302,19 @@ __device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
     if (x >= P.nx || y >= P.ny) return;
     const size_t i = (size_t)s * P.nx * P.ny + (size_t)y * P.nx + x;
     const float ktl = (PART & PRIMAL_U) ? P.ktl[i] : 0.f;
-    primal_update<DIM, TGV, PART>(P, i, x, y, s % P.nz, ktl);
+    primal_update<DIM, TGV, PART, ZH>(P, i, x, y, s % P.nz, ktl);
 }
 
 // PART selects the u half (needs K^T lam), the p half (needs only v and w,
 // so it can run beside the projections) or both.
-template <int DIM, bool TGV, int PART>
+template <int DIM, bool TGV, int PART, bool ZH>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_single_kernel(ProbDev P, int tiles_x,
                                                                          int tiles_y)
 {
     const int b = stencil_block<PRIMAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    primal_body<DIM, TGV, PART>(P, I);
+    primal_body<DIM, TGV, PART, ZH>(P, I);
 }
 
 template <int DIM, bool TGV>
@@ -646,19 +646,30 @@ int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st)
     }
 }
 
+template <int DIM, bool TGV, bool ZH>
+static void primal_single_part_zh(const ProbDev& P, int part, dim3 grd, dim3 blk, int tiles_x,
+                                  int tiles_y, hipStream_t st)
+{
+    if (part == PRIMAL_U)
+        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_U, ZH>), grd, blk, 0, st, P,
+                           tiles_x, tiles_y);
+    else if (part == PRIMAL_P)
+        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_P, ZH>), grd, blk, 0, st, P,
+                           tiles_x, tiles_y);
+    else
+        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_UP, ZH>), grd, blk, 0, st, P,
+                           tiles_x, tiles_y);
+}
+
+// a z-slab with halo planes takes the kernels that read them
 template <int DIM, bool TGV>
 static void primal_single_part(const ProbDev& P, int part, dim3 grd, dim3 blk, int tiles_x,
                                int tiles_y, hipStream_t st)
 {
-    if (part == PRIMAL_U)
-        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_U>), grd, blk, 0, st, P, tiles_x,
-                           tiles_y);
-    else if (part == PRIMAL_P)
-        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_P>), grd, blk, 0, st, P, tiles_x,
-                           tiles_y);
+    if (DIM == 3 && (P.halo_ubar_hi || P.halo_v_lo))
+        primal_single_part_zh<DIM, TGV, DIM == 3>(P, part, grd, blk, tiles_x, tiles_y, st);
     else
-        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_UP>), grd, blk, 0, st, P,
-                           tiles_x, tiles_y);
+        primal_single_part_zh<DIM, TGV, false>(P, part, grd, blk, tiles_x, tiles_y, st);
 }
 
 int launch_primal(const ProbDev& P, hipStream_t st) { return launch_primal_part(P, PRIMAL_UP, st); }

@@ -35,14 +35,22 @@ __device__ __forceinline__ float lane_next(float v)
 // (lane = x - x0, every lane below the last active one active), so the x - 1
 // and x + 1 neighbours of v and w come from the neighbouring lanes and only
 // the wave's edge lanes read them from memory.
-template <int DIM, bool TGV, bool NEED_U, bool NEED_P, bool LANEX = false>
+template <int DIM, bool TGV, bool NEED_U, bool NEED_P, bool LANEX = false, bool ZH = false>
 __device__ __forceinline__ void primal_point(
     const float* __restrict__ u, const float* __restrict__ p, const float* __restrict__ v,
     const float* __restrict__ w, float ktl, int nx, int ny, int nz, size_t cs, float hinv,
     float tau, float norming, size_t i, int x, int y, int z, float& un, float& ub,
-    float (&pn)[3], float (&pb)[3])
+    float (&pn)[3], float (&pb)[3], const ProbDev* zh = nullptr)
 {
     const size_t plane = (size_t)nx * ny;
+    // slab of a larger volume (zh's halo planes): planes exist beyond z = 0 /
+    // z = nz - 1, which then are not the volume's boundary
+    const bool has_lo = ZH && zh->halo_v_lo, has_hi = ZH && zh->halo_ubar_hi;
+    // offset of (component comp, this channel, y, x) in a halo plane set
+    const size_t hc = ZH ? (i / plane) / (size_t)nz : 0;  // channel of i = [c][z][y][x]
+    auto halo_off = [&](int comp) {
+        return ((size_t)comp * zh->nch + hc) * plane + (size_t)y * nx + x;
+    };
     float vc[3];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) vc[k] = v[i + k * cs];
@@ -60,8 +68,14 @@ __device__ __forceinline__ void primal_point(
         if (y > 0) d1 -= v[i + cs - nx];
         float dv = d0 + d1;
         if constexpr (DIM == 3) {
-            float d2 = (z == nz - 1) ? 0.f : vc[2];
-            if (z > 0) d2 -= v[i + 2 * cs - plane];
+            float d2 = (z == nz - 1 && !has_hi) ? 0.f : vc[2];
+            if constexpr (ZH) {
+                const float* dn = z > 0 ? v + i + 2 * cs - plane
+                                  : has_lo ? zh->halo_v_lo + halo_off(0) : nullptr;
+                if (dn) d2 -= *dn;
+            } else {
+                if (z > 0) d2 -= v[i + 2 * cs - plane];
+            }
             dv += d2 * hinv;
         }
         const float uo = u[i];
@@ -105,8 +119,14 @@ __device__ __forceinline__ void primal_point(
                 }
                 float ys = (y == 0) ? 0.f : -wc[yi[k]];
                 if (y < ny - 1) ys += w[i + yi[k] * cs + nx];
-                float zs = (z == 0) ? 0.f : -wc[zi[k]];
-                if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
+                float zs = (z == 0 && !has_lo) ? 0.f : -wc[zi[k]];
+                if constexpr (ZH) {
+                    const float* up = z < nz - 1 ? w + i + zi[k] * cs + plane
+                                      : has_hi ? zh->halo_w_hi + halo_off(k) : nullptr;
+                    if (up) zs += *up;
+                } else {
+                    if (z < nz - 1) zs += w[i + zi[k] * cs + plane];
+                }
                 q[k] = xs + ys + zs * hinv;
             }
         }

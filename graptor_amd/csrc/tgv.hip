@@ -179,7 +179,7 @@ struct FusedSrc {
 // sigma E pbar), written to v_out / w_out (the same arrays as v / w for the
 // standalone step: each thread reads its own entries before writing them,
 // so they are not declared restrict).
-template <int DIM, int CPL, bool TGV, int MAXC, class Src>
+template <int DIM, int CPL, bool TGV, int MAXC, class Src, bool ZH = false>
 __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z, const Src& src,
                                           const float* v, const float* w, float* vout,
                                           float* wout)
@@ -213,8 +213,18 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
             g[0] = (x < nx - 1) ? src.ubn(c, ic + 1, x + 1, y, z) - u0 : 0.f;
         }
         g[1] = (y < ny - 1) ? src.ubn(c, ic + nx, x, y + 1, z) - u0 : 0.f;
-        if constexpr (DIM == 3)
-            g[2] = (z < nz - 1) ? (src.ubn(c, ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
+        if constexpr (DIM == 3) {
+            if constexpr (ZH) {
+                // stored ubar; above a slab's last plane the halo plane, if any
+                const float* up = z < nz - 1 ? src.ubar + ic + plane
+                                  : P.halo_ubar_hi
+                                      ? P.halo_ubar_hi + (size_t)c * plane + (size_t)y * nx + x
+                                      : nullptr;
+                g[2] = up ? (*up - u0) * hinv : 0.f;
+            } else {
+                g[2] = (z < nz - 1) ? (src.ubn(c, ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             vv[c][k] = v[ic + k * cs] + sigma * (g[k] - p[k]);
@@ -242,7 +252,19 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
                 for (int k = 0; k < D; ++k) ay[k] = p[k] - q[k];
             }
             if constexpr (DIM == 3) {
-                if (z > 0) {
+                if constexpr (ZH) {
+                    // stored pbar; below a slab's first plane the halo plane, if any
+                    const float* dn = z > 0 ? src.pbar + ic - plane
+                                      : P.halo_pbar_lo
+                                          ? P.halo_pbar_lo + (size_t)c * plane +
+                                                (size_t)y * nx + x
+                                          : nullptr;
+                    const size_t kst = z > 0 ? cs : (size_t)nch * plane;
+                    if (dn) {
+#pragma unroll
+                        for (int k = 0; k < D; ++k) az[k] = (p[k] - dn[k * kst]) * hinv;
+                    }
+                } else if (z > 0) {
                     src.pbn(c, ic - plane, x, y, z - 1, q);
 #pragma unroll
                     for (int k = 0; k < D; ++k) az[k] = (p[k] - q[k]) * hinv;
@@ -372,27 +394,28 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
     }
 }
 
-template <int DIM, int CPL, bool TGV, int MAXC>
+// ZH: the problem is a z-slab with halo planes (gr_tgv_state's halo_*)
+template <int DIM, int CPL, bool TGV, int MAXC, bool ZH = false>
 __device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
 {
     const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
     const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
     const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;
     const MemSrc<DIM, TGV> src{P.ubar, P.pbar, cs};
-    dual_core<DIM, CPL, TGV, MAXC>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
+    dual_core<DIM, CPL, TGV, MAXC, MemSrc<DIM, TGV>, ZH>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
 }
 
 // Work order: x tiles fastest, then rows, then planes, with runs of
 // DUAL_GROUP tiles kept on one XCD (plane-major orders, z-fastest orders and
 // giving each XCD a contiguous eighth of the volume all measured slower).
-template <int DIM, int CPL, bool TGV, int MAXC>
+template <int DIM, int CPL, bool TGV, int MAXC, bool ZH>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_single_kernel(ProbDev P, int tiles_x,
                                                                        int tiles_y)
 {
     const int b = stencil_block<DUAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    dual_body<DIM, CPL, TGV, MAXC>(P, I);
+    dual_body<DIM, CPL, TGV, MAXC, ZH>(P, I);
 }
 
 template <int DIM, int CPL, bool TGV, int MAXC>
@@ -551,8 +574,13 @@ static void dual_single_launch(const ProbDev& P, hipStream_t st)
     const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
     const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
     const long long blocks = (long long)tiles_x * tiles_y * P.nz;
-    hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC>), dim3((unsigned)blocks),
-                       dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+    if (DIM == 3 && (P.halo_ubar_hi || P.halo_v_lo))
+        hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, DIM == 3>),
+                           dim3((unsigned)blocks), dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x,
+                           tiles_y);
+    else
+        hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, false>), dim3((unsigned)blocks),
+                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
 }
 
 // Primal step of iteration k fused with the dual step of iteration k + 1:

@@ -116,6 +116,17 @@ typedef struct gr_tgv_state {
     float *lam, *ktl, *ku;
     const float *f, *mfac;
     float *u2, *v2, *w2;
+    /* Optional z halo (3D) for a z-slab of a larger volume, as one rank of a
+     * slab-sharded solve would hold it; NULL: the slab's first / last plane
+     * is the volume's boundary.  Planes are [c][y][x] of the neighbouring
+     * slab's adjacent plane: halo_ubar_hi = ubar above (C planes),
+     * halo_pbar_lo = pbar below ([k][c], 3 C planes), halo_v_lo = v_z below
+     * (C planes), halo_w_hi = w_xz, w_yz, w_zz above ([j][c], 3 C planes).
+     * No counterpart in the reference (it solves slabs independently,
+     * RC:408-458); the dual step reads halo_ubar_hi / halo_pbar_lo, the
+     * primal step halo_v_lo / halo_w_hi.  Not supported with u2/v2/w2 or a
+     * geometry that fuses the primal step. */
+    const float *halo_ubar_hi, *halo_pbar_lo, *halo_v_lo, *halo_w_hi;
 } gr_tgv_state;
 
 /* Runs n_iter iterations of the loop 2D:767-799 / 3D:1194-1221 on `stream`:

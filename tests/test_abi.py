@@ -61,4 +61,4 @@ def test_invalid_arguments_are_reported_not_raised():
 def test_param_struct_layout_matches_header():
     # 8 ints then 7 floats, no padding
     assert ctypes.sizeof(_lib.TgvParams) == 15 * 4
-    assert ctypes.sizeof(_lib.TgvState) == 14 * ctypes.sizeof(ctypes.c_void_p)
+    assert ctypes.sizeof(_lib.TgvState) == 18 * ctypes.sizeof(ctypes.c_void_p)
