@@ -463,9 +463,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_batch_kernel(const ProbDev* _
 // --------------------------------------------------------------------------
 // sizes
 // --------------------------------------------------------------------------
-int fwd_table_choice(const Geometry& g, long long groups)
+int fwd_table_choice(const Geometry& g, long long groups, double wg_per_cu)
 {
-    constexpr double target = 3.0 * 256;  // work-groups: ~3 per CU on 256 CUs
+    const double target = wg_per_cu * 256;  // work-groups per CU on 256 CUs
     int best = 0;
     double best_d = 1e30;
     for (int t = 0; t < 4; ++t) {
@@ -551,8 +551,12 @@ static int fwd_single_k(const ProbDev& Pin, const Geometry& g, int nslices, int 
     ProbDev P = Pin;
     const int groups = (nslices + K - 1) / K;
     // chunk table: items per thread (8, 4, 2, 1) whose grid is closest to
-    // three work-groups per CU (GRAPTOR_FWD_MI = 8, 4, 2 or 1 forces it)
-    int t = fwd_table_choice(g, groups);
+    // three work-groups per CU, or 1.5 beside the dual step, where fewer,
+    // larger work-groups leave it more of the CUs (measured: 0.827 against
+    // 0.856 ms per two-stream iteration on 256x256x64x4, although the
+    // forward projection alone is slower); GRAPTOR_FWD_MI = 8, 4, 2 or 1
+    // forces the table
+    int t = fwd_table_choice(g, groups, (mode & FWD_BESIDE) ? 1.5 : 3.0);
     static const int forced = [] {
         const char* e = getenv("GRAPTOR_FWD_MI");
         const int v = e ? atoi(e) : 0;
