@@ -22,6 +22,7 @@ COUPLING = {"uncorrelated": 0, "frobenius": 1, "nuclear": 2}
 DISCREPANCY = {"KL": 0, "L2": 1}
 REGULARISATION = {"TV": 0, "TGV": 1}
 NORMEST_MODE = {"reference": 0, "power": 1}
+HALO_PHASE = {"dual": 0, "primal": 1}
 
 # every symbol the header declares (checked by the CPU test suite)
 EXPORTS = (
@@ -34,7 +35,7 @@ EXPORTS = (
     "gr_cnn_conv_fwd", "gr_cnn_conv_wgrad", "gr_cnn_bias_grad", "gr_cnn_weight_flip",
     "gr_cnn_relu_bwd", "gr_cnn_pool", "gr_cnn_upsample", "gr_cnn_pack_bf16", "gr_cnn_mse_grad",
     "gr_cnn_adam", "gr_tgv_dual_step", "gr_tgv_primal_step", "gr_csr_apply", "gr_prox",
-    "gr_sumsq_f64",
+    "gr_sumsq_f64", "gr_tgv_pack_halos",
 )
 
 
@@ -127,6 +128,8 @@ def load(path: str | None = None):
             "gr_csr_apply": ([V, V, V, I, I, I, I, I, I, V, V, V], I),
             "gr_prox": ([I, F, V, I, S, V, V, V, V], I),
             "gr_sumsq_f64": ([V, S, PD, V], I),
+            "gr_tgv_pack_halos": ([ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState), I, V, V,
+                                   V], I),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

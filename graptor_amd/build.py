@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["api.hip", "projector.hip", "tgv.hip", "aux.hip", "cnn.hip", "subsample.hip"]
+SOURCES = ["api.hip", "projector.hip", "tgv.hip", "aux.hip", "cnn.hip", "subsample.hip", "halo.hip"]
 
 
 def hipcc() -> str:

@@ -86,15 +86,10 @@ bool valid_params(const gr_geometry* g, const gr_tgv_params* p)
     return true;
 }
 
-bool valid_state(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_state* s,
-                 bool need_ku)
+// A halo side is all of its planes (the p / w ones only with TGV), 3D only.
+bool valid_halos(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_state* s)
 {
     const bool tgv = p->regularisation == GR_REG_TGV;
-    if (!s || !s->u || !s->ubar || !s->v || !s->lam || !s->f || !s->mfac) return false;
-    if (!g->g.fuse_primal && !s->ktl) return false;
-    if (tgv && (!s->p || !s->pbar || !s->w)) return false;
-    if (need_ku && !s->ku) return false;
-    // a halo side is all of its planes (the p / w ones only with TGV)
     const bool lo = s->halo_v_lo != nullptr, hi = s->halo_ubar_hi != nullptr;
     if ((lo || hi) && p->dim != 3) return false;
     if (!lo && s->halo_pbar_lo) return false;
@@ -103,6 +98,17 @@ bool valid_state(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_stat
     if (tgv && hi && !s->halo_w_hi) return false;
     if ((lo || hi) && (g->g.fuse_primal || s->u2)) return false;
     return true;
+}
+
+bool valid_state(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_state* s,
+                 bool need_ku)
+{
+    const bool tgv = p->regularisation == GR_REG_TGV;
+    if (!s || !s->u || !s->ubar || !s->v || !s->lam || !s->f || !s->mfac) return false;
+    if (!g->g.fuse_primal && !s->ktl) return false;
+    if (tgv && (!s->p || !s->pbar || !s->w)) return false;
+    if (need_ku && !s->ku) return false;
+    return valid_halos(g, p, s);
 }
 
 // Descriptor of one problem with the geometry's tuning applied for its
@@ -733,8 +739,8 @@ int gr_tgv_dual_step(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv
                      void* stream)
 {
     if (!valid_params(gg, p) || !s || !s->ubar || !s->v ||
-        (p->regularisation == GR_REG_TGV && (!s->pbar || !s->w))) {
-        set_error("tgv_dual_step: invalid parameters or missing state buffer");
+        (p->regularisation == GR_REG_TGV && (!s->pbar || !s->w)) || !valid_halos(gg, p, s)) {
+        set_error("tgv_dual_step: invalid parameters, missing state buffer or halo planes");
         return GR_ERR_INVALID;
     }
     const ProbDev P = make_prob(gg, p, s);
@@ -745,8 +751,9 @@ int gr_tgv_primal_step(const gr_geometry* gg, const gr_tgv_params* p, const gr_t
                        void* stream)
 {
     if (!valid_params(gg, p) || !s || !s->u || !s->ubar || !s->v || !s->ktl ||
-        (p->regularisation == GR_REG_TGV && (!s->p || !s->pbar || !s->w))) {
-        set_error("tgv_primal_step: invalid parameters or missing state buffer");
+        (p->regularisation == GR_REG_TGV && (!s->p || !s->pbar || !s->w)) ||
+        !valid_halos(gg, p, s)) {
+        set_error("tgv_primal_step: invalid parameters, missing state buffer or halo planes");
         return GR_ERR_INVALID;
     }
     const ProbDev P = make_prob(gg, p, s);

@@ -8,13 +8,14 @@ channels).  Per iteration a slab's dual step needs ubar of the plane above
 and pbar of the plane below it, its primal step v_z of the plane below and
 (w_xz, w_yz, w_zz) of the plane above: the halo planes of gr_tgv_state.
 Ranks would exchange these planes with their neighbours twice per iteration
-(send / receive pairs over xGMI); here all slabs share one GPU and the
-planes are device-to-device copies, so the result equals the unsplit
-volume's solve bit for bit.
+(send / receive pairs over xGMI) as buffers packed by gr_tgv_pack_halos;
+here all slabs share one GPU and each slab packs its planes straight into
+its neighbours' halo fields, so the result equals the unsplit volume's solve
+bit for bit.
 
 The iteration is composed from the single-step entry points (dual step,
 forward projection, prox, backprojection, primal step) so that the halo
-copies can sit between the dual and the primal steps; the reference solves
+exchanges can sit between the dual and the primal steps; the reference solves
 slabs as independent problems instead (RC:408-458, with overlapping slabs).
 """
 from __future__ import annotations
@@ -27,10 +28,6 @@ import torch
 from . import _lib
 from .projector import RadonGeometry, radon, radon_ad
 from .solver import TgvProblem
-
-# w components of the z-part of div2 w, in the order of gr_tgv_state.halo_w_hi
-_W_Z = (4, 5, 2)
-
 
 class SlabSolve:
     """``nslabs`` z-slabs of a (C, Nz, na, nd) data stack, each a
@@ -72,25 +69,31 @@ class SlabSolve:
             p.state.halo_w_hi = ptr("w_hi")
             self.halos.append(h)
 
-    # halo exchanges (a rank's sends / receives; here plane copies)
+    # halo exchanges: each slab packs its outgoing planes with one launch
+    # (gr_tgv_pack_halos); a rank would send those buffers to its neighbours and
+    # receive into its halo fields.  Slabs sharing one GPU pack straight into
+    # the neighbours' halo fields.
+    def _pack(self, phase: str, below: str, above: str) -> None:
+        lib = _lib.load()
+        for s, p in enumerate(self.slabs):
+            to_below = self.halos[s - 1].get(below) if s > 0 else None
+            to_above = self.halos[s + 1].get(above) if s + 1 < len(self.slabs) else None
+            if to_below is None and to_above is None:
+                continue
+            _lib.check(lib.gr_tgv_pack_halos(
+                ctypes.byref(p.params), ctypes.byref(p.state), _lib.HALO_PHASE[phase],
+                to_below.data_ptr() if to_below is not None else None,
+                to_above.data_ptr() if to_above is not None else None, p._stream()),
+                "pack_halos")
+
     def exchange_dual_halos(self):
         """ubar of the plane above, pbar of the plane below (read by the dual step)."""
-        for s, h in enumerate(self.halos):
-            if "ubar_hi" in h:
-                h["ubar_hi"].copy_(self.slabs[s + 1].ubar[:, 0])
-            if "pbar_lo" in h:
-                h["pbar_lo"].copy_(self.slabs[s - 1].pbar[:, :, -1])
+        self._pack("dual", "ubar_hi", "pbar_lo")
 
     def exchange_primal_halos(self):
         """v_z of the plane below, (w_xz, w_yz, w_zz) of the plane above (read
         by the primal step)."""
-        for s, h in enumerate(self.halos):
-            if "v_lo" in h:
-                h["v_lo"].copy_(self.slabs[s - 1].v[2, :, -1])
-            if "w_hi" in h:
-                up = self.slabs[s + 1].w
-                for j, m in enumerate(_W_Z):
-                    h["w_hi"][j].copy_(up[m, :, 0])
+        self._pack("primal", "w_hi", "v_lo")
 
     def iterate(self, n: int) -> None:
         lib = _lib.load()

@@ -159,6 +159,23 @@ int gr_tgv_dual_step(const gr_geometry *g, const gr_tgv_params *prm, const gr_tg
 int gr_tgv_primal_step(const gr_geometry *g, const gr_tgv_params *prm, const gr_tgv_state *st,
                        void *stream);
 
+/* Halo messages of a z-slab (3D; no counterpart in the reference, which
+ * solves slabs independently, RC:408-458): packs the planes this slab sends
+ * to its neighbours, each as one contiguous buffer laid out as the
+ * receiver's gr_tgv_state halo field, so a rank sends it as one message and
+ * receives into its own halo field with no unpack step.
+ *   GR_HALO_DUAL (before the dual step): to_below <- ubar of plane 0
+ *     (the lower slab's halo_ubar_hi, nch planes); to_above <- pbar of plane
+ *     nz-1 (the upper slab's halo_pbar_lo, 3 nch planes, TGV only).
+ *   GR_HALO_PRIMAL (before the primal step): to_below <- w_xz, w_yz, w_zz of
+ *     plane 0 (halo_w_hi, 3 nch planes, TGV only); to_above <- v_z of plane
+ *     nz-1 (halo_v_lo, nch planes).
+ * A NULL destination (a slab at the volume's boundary) is skipped; one
+ * launch on `stream` packs both. */
+enum { GR_HALO_DUAL = 0, GR_HALO_PRIMAL = 1 };
+int gr_tgv_pack_halos(const gr_tgv_params *prm, const gr_tgv_state *st, int phase,
+                      float *to_below, float *to_above, void *stream);
+
 /* Primal energy at (u, p) reduced in FP64 on the device (no counterpart in
  * the reference's hot path; its Gap_TGV_KL 3D:882-927 is unused):
  *   KL: sum_c mu_c KL(R u_c, f_c) + TGV_{alpha/L}(u; p)

@@ -58,6 +58,27 @@ def test_invalid_arguments_are_reported_not_raised():
         _lib.check(_lib.GR_ERR_OOM, "x")
 
 
+def test_pack_halos_rejects_what_a_slab_does_not_send():
+    """gr_tgv_pack_halos checks its arguments on the host before any launch:
+    3D only, a known phase, no pbar / w message without TGV; with both
+    destinations NULL it has nothing to do."""
+    lib = _lib.load()
+    prm = _lib.TgvParams(dim=3, nx=8, ny=8, nz=4, nch=2, coupling=1, discrepancy=0,
+                         regularisation=_lib.REGULARISATION["TV"], stepsize=1.0)
+    fake = 1 << 20  # never dereferenced: every call below returns before a launch
+    st = _lib.TgvState(ubar=fake, v=fake)
+    P, S = ctypes.byref(prm), ctypes.byref(st)
+    dual, primal = _lib.HALO_PHASE["dual"], _lib.HALO_PHASE["primal"]
+    assert lib.gr_tgv_pack_halos(P, S, dual, None, fake, None) == _lib.GR_ERR_INVALID
+    assert b"TGV only" in lib.gr_last_error()
+    assert lib.gr_tgv_pack_halos(P, S, primal, fake, None, None) == _lib.GR_ERR_INVALID
+    assert lib.gr_tgv_pack_halos(P, S, 2, None, None, None) == _lib.GR_ERR_INVALID
+    assert lib.gr_tgv_pack_halos(P, S, dual, None, None, None) == _lib.GR_OK
+    prm.dim, prm.nz = 2, 1
+    assert lib.gr_tgv_pack_halos(P, S, dual, None, None, None) == _lib.GR_ERR_INVALID
+    assert lib.gr_tgv_pack_halos(None, S, dual, None, None, None) == _lib.GR_ERR_INVALID
+
+
 def test_param_struct_layout_matches_header():
     # 8 ints then 7 floats, no padding
     assert ctypes.sizeof(_lib.TgvParams) == 15 * 4

@@ -84,3 +84,53 @@ def test_halo_state_validation():
     with pytest.raises(ValueError):
         p.iterate(1)
     assert _lib is not None
+
+
+def _pack(prob, phase, below, above):
+    import ctypes
+
+    from graptor_amd import _lib
+    lib = _lib.load()
+    return lib.gr_tgv_pack_halos(ctypes.byref(prob.params), ctypes.byref(prob.state),
+                                 _lib.HALO_PHASE[phase],
+                                 below.data_ptr() if below is not None else None,
+                                 above.data_ptr() if above is not None else None, prob._stream())
+
+
+@pytest.mark.parametrize("N,offset", [(40, 0), (39, 0), (40, 1)],
+                         ids=["float4", "odd-plane", "unaligned"])
+def test_pack_halos_equal_the_edge_planes(N, offset):
+    """gr_tgv_pack_halos writes exactly the planes a neighbour's halo fields
+    hold (ubar of plane 0, pbar of the last plane; w_xz, w_yz, w_zz of plane
+    0, v_z of the last plane), in the [component][channel] order, for the
+    float4 and the scalar copy paths; a NULL side leaves its buffer alone."""
+    from graptor_amd import _lib
+    from graptor_amd.solver import TgvProblem
+    print(f"pack_halos N={N} offset={offset}", flush=True)
+    f, g, L = _setup(seed=64, Nz=5, N=N)
+    prob = TgvProblem(f, g, (4, 1), [1.0, 0.3, 0.1], dim=3, L=L)
+    gen = torch.Generator(device=prob.device).manual_seed(7)
+    for name in ("ubar", "pbar", "v", "w"):
+        t = getattr(prob, name)
+        t.copy_(torch.randn(t.shape, generator=gen, device=t.device))
+    C = prob.C
+
+    def buf(ncomp):
+        n = ncomp * C * N * N
+        raw = torch.full((n + offset,), float("nan"), device=prob.device)
+        return raw[offset:].view(ncomp, C, N, N)
+
+    d_lo, d_hi, p_lo, p_hi = buf(1), buf(3), buf(3), buf(1)
+    _lib.check(_pack(prob, "dual", d_lo, d_hi), "pack dual")
+    _lib.check(_pack(prob, "primal", p_lo, p_hi), "pack primal")
+    torch.cuda.synchronize()
+    assert torch.equal(d_lo[0], prob.ubar[:, 0])
+    assert torch.equal(d_hi, prob.pbar[:, :, -1])
+    assert torch.equal(p_lo, torch.stack([prob.w[m, :, 0] for m in (4, 5, 2)]))
+    assert torch.equal(p_hi[0], prob.v[2, :, -1])
+    # one side only (a slab at the volume's edge)
+    lone = buf(1)
+    _lib.check(_pack(prob, "primal", None, lone), "pack primal above")
+    _lib.check(_pack(prob, "dual", None, None), "pack nothing")
+    torch.cuda.synchronize()
+    assert torch.equal(lone[0], prob.v[2, :, -1])
