@@ -116,6 +116,10 @@ constexpr int ADJ_MAXR = 8;          // rows per thread (tile_y <= 32)
 #define GR_DUAL_BY 4
 #endif
 constexpr int DUAL_BX = GR_DUAL_BX, DUAL_BY = GR_DUAL_BY;
+// the stencils take x +- 1 neighbours from the adjacent lanes of a wave
+// (wave-wide shifts with memory loads at lanes 0 and 63): a tile row must be
+// exactly one wave
+static_assert(DUAL_BX == 64, "stencil tiles are one 64-lane wave wide");
 
 #ifndef GR_DUAL_GROUP
 #define GR_DUAL_GROUP 8

@@ -614,8 +614,9 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void pd_fused_kernel(ProbDev P, F
 constexpr int FZ_RUN = 8;
 constexpr int FZ_W = DUAL_BX + 1;
 constexpr int FZ_H = DUAL_BY + 1;
-static_assert(2 * (DUAL_BX + DUAL_BY) <= DUAL_BX * DUAL_BY,
-              "the halo of a tile needs one thread per halo entry");
+// the halo of a tile is loaded with one thread per halo entry; smaller tiles
+// (measurement builds) run the 3D fused step with pd_fused_kernel instead
+constexpr bool FZ_FITS = 2 * (DUAL_BX + DUAL_BY) <= DUAL_BX * DUAL_BY;
 
 template <int MAXC, bool TGV>
 struct FzPlane {
@@ -800,7 +801,7 @@ static void fused_z_launch(const ProbDev& P, const FusedBufs& B, hipStream_t st)
 template <int DIM, int CPL, bool TGV, int MAXC>
 static void fused_launch(const ProbDev& P, const FusedBufs& B, hipStream_t st)
 {
-    if constexpr (DIM == 3) {
+    if constexpr (DIM == 3 && FZ_FITS) {
         fused_z_launch<CPL, TGV, MAXC>(P, B, st);
         return;
     }
