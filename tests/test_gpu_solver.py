@@ -220,6 +220,28 @@ def test_two_stream_iteration_matches_one_stream(combo):
             assert torch.equal(a, b), (n, name)
 
 
+def test_two_streams_with_many_slice_groups():
+    """2048 slices (512 slice groups of 4, a forward grid of over a thousand
+    work-groups beside the dual step): the two-stream iterations equal the
+    one-stream order bit for bit."""
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    C, Nz, N, na = 4, 512, 64, 60
+    print(f"many slice groups C={C} Nz={Nz} N={N} na={na}", flush=True)
+    rng = np.random.default_rng(71)
+    f = rng.poisson(20.0, size=(C, Nz, na, N)).astype(np.float32)
+    ang = np.deg2rad(np.linspace(-75, 75, na)) + np.pi / 2
+    g = RadonGeometry((N, N), ang, n_detectors=N)
+    kw = dict(dim=3, coupling="frobenius", discrepancy="KL", regularisation="TGV", L=90.0)
+    two = TgvProblem(f, g, (4, 1), [0.2, 5e-4, 4e-3, 5e-4], **kw)
+    one = TgvProblem(f, g, (4, 1), [0.2, 5e-4, 4e-3, 5e-4], **kw)
+    two.iterate(3)
+    one.profile(3)
+    torch.cuda.synchronize()
+    for name in ("u", "p", "v", "w", "lam"):
+        assert torch.equal(getattr(two, name), getattr(one, name)), name
+
+
 def test_two_streams_from_two_host_threads():
     """Each host thread gets its own side stream: two problems iterated from
     two threads on their own torch streams give the same bits as the same
