@@ -7,8 +7,10 @@ Workload (one "step" = one primal-dual iteration over the whole volume):
 granule phantom, 60 tilts in [-75, 75] degrees, 256 detector bins, Poisson
 data, alpha = (4, 1), mu = (0.2, 5e-4, 4e-3, 5e-4), FP32 state, inputs
 resident in HBM.  Prints one JSON line with voxel-iters/s, the roofline of
-the dominant kernel (HIP events over its launches) and a CPU baseline (the
-FP32 C/OpenMP restatement of the reference kernels on this host's cores).
+the dominant kernel (HIP events over its launches), the same byte rates for
+the two projectors (vector-ALU and LDS bound, far below the HBM peak) and a
+CPU baseline (the FP32 C/OpenMP restatement of the reference kernels on this
+host's cores).
 """
 from __future__ import annotations
 
@@ -599,15 +601,26 @@ def main():
                     "memory_GiB": torch.cuda.max_memory_allocated(dev) / 2 ** 30}
         print(f"# converge: {converge}", file=sys.stderr, flush=True)
 
-    traffic = None
+    counted = {}
     tpath = os.path.join(ROOT, "profiles", "coupled3d_traffic.json")
     if not large and args.nz == 64 and os.path.exists(tpath):
         # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
-        # (scaled by a 1 GiB copy) for this workload
+        # (scaled by a 1 GiB copy) for this workload, one-stream launches
         with open(tpath) as fh:
-            tk = json.load(fh)["kernels_MiB_per_launch"].get(dom)
-        if tk:
-            traffic = (tk["fetch"] + tk["write"]) * 2 ** 20
+            table = json.load(fh)["kernels_MiB_per_launch"]
+        for k, key in (("dual_single_kernel", "dual_single_kernel"),
+                       ("fwd_single_kernel", "fwd_single_kernel_profiled"),
+                       ("adj_single_kernel", "adj_single_kernel"),
+                       ("primal_single_kernel", "primal_single_kernel")):
+            if key in table:
+                counted[k] = (table[key]["fetch"] + table[key]["write"]) * 2 ** 20
+    traffic = counted.get(dom)
+    # the projectors, by the same algorithmic-bytes rule: bound by vector ALU
+    # and LDS work, so their byte rate is far below the HBM peak
+    projector = {k: {"avg_launch_ms": kern[k]["avg_ms"], "bytes_per_launch": kern[k]["bytes"],
+                     "achieved": kern[k]["gbs"], "unit": "GB/s",
+                     "frac": kern[k]["gbs"] / HBM_PEAK_GBS, "traffic": counted.get(k)}
+                 for k in ("fwd_single_kernel", "adj_single_kernel") if k in kern}
 
     line = {
         "metric": "voxel-iters/s (coupled TGV-KL) + time-to-1e-3 rel err: warm-start vs plain PD",
@@ -632,6 +645,7 @@ def main():
                      "bytes_per_launch": kern[dom]["bytes"],
                      "avg_launch_ms": kern[dom]["avg_ms"]},
         "kernels_ms": {k: v["avg_ms"] for k, v in kern.items()},
+        "projector": dict(bound="valu+lds", **projector),
         "cpu_baseline": cpu,
     }
     if converge is not None:
