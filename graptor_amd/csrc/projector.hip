@@ -331,9 +331,10 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_batch_kernel(
 // chunk of angles it stages, for each angle, only the sinogram segment the
 // tile projects onto (bins outside [0, nd) staged as zeros, which reproduces
 // the reference's edge guards), then every thread interpolates for its
-// tile_y/4 pixels.  PRIMAL fuses the primal step into the epilogue.
+// tile_y/4 pixels, tile_y <= 4 * MAXR (the accumulators a lane holds: 8 rows
+// take 126 VGPRs, 4 rows 70).  PRIMAL fuses the primal step into the epilogue.
 // --------------------------------------------------------------------------
-template <int K, bool PRIMAL, int DIM, bool TGV>
+template <int K, bool PRIMAL, int DIM, bool TGV, int MAXR = ADJ_MAXR>
 __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, float* sh)
 {
     const int nx = P.nx, ny = P.ny, nd = P.nd, na = P.na;
@@ -350,9 +351,9 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
     const float xf = (float)(x0 + lane);
     const float* __restrict__ sino = P.lam;
 
-    float acc[ADJ_MAXR][K];
+    float acc[MAXR][K];
 #pragma unroll
-    for (int r = 0; r < ADJ_MAXR; ++r)
+    for (int r = 0; r < MAXR; ++r)
 #pragma unroll
         for (int k = 0; k < K; ++k) acc[r][k] = 0.f;
 
@@ -398,7 +399,7 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
             const float tb = fmaf(xf, pa.x, pa.z);
             const int base = al * segw;
 #pragma unroll
-            for (int r = 0; r < ADJ_MAXR; ++r) {
+            for (int r = 0; r < MAXR; ++r) {
                 if (r >= R) break;
                 // t >= 1 over the whole tile (the segment starts at
                 // floor(min t) - 1), so truncation is the floor
@@ -419,7 +420,7 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
     if (x >= nx) return;
     const size_t plane = (size_t)nx * ny;
 #pragma unroll
-    for (int r = 0; r < ADJ_MAXR; ++r) {
+    for (int r = 0; r < MAXR; ++r) {
         if (r >= R) break;
         const int y = y0 + wv + 4 * r;
         if (y >= ny) continue;
@@ -436,7 +437,7 @@ __device__ __forceinline__ void adj_body(const ProbDev& P, const AdjItem& I, flo
     }
 }
 
-template <int K, bool PRIMAL, int DIM, bool TGV>
+template <int K, bool PRIMAL, int DIM, bool TGV, int MAXR>
 __global__ __launch_bounds__(ADJ_THREADS) void adj_single_kernel(ProbDev P, int tiles_x,
                                                                  int ntiles)
 {
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_single_kernel(ProbDev P, int 
     const int tile = work % ntiles;
     const AdjItem I{0, (tile % tiles_x) * ADJ_TX, (tile / tiles_x) * P.adj_tile_y,
                     (work / ntiles) * K};
-    adj_body<K, PRIMAL, DIM, TGV>(P, I, sh);
+    adj_body<K, PRIMAL, DIM, TGV, MAXR>(P, I, sh);
 }
 
 template <int K, bool PRIMAL, int DIM, bool TGV>
@@ -601,7 +602,18 @@ template <int K, bool PRIMAL, int DIM, bool TGV>
 static void adj_single_launch(const ProbDev& P, int blocks, int tiles_x, int ntiles, size_t lds,
                               hipStream_t st)
 {
-    hipLaunchKernelGGL((adj_single_kernel<K, PRIMAL, DIM, TGV>), dim3((unsigned)blocks),
+    // the backprojection alone takes accumulators for the rows of its tile
+    // (tiles of <= 16 rows: 7 instead of 4 waves per SIMD); the fused primal
+    // epilogue keeps the 8-row form
+    if constexpr (!PRIMAL) {
+        if (P.adj_tile_y <= 16) {
+            hipLaunchKernelGGL((adj_single_kernel<K, PRIMAL, DIM, TGV, 4>),
+                               dim3((unsigned)blocks), dim3(ADJ_THREADS), lds, st, P, tiles_x,
+                               ntiles);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((adj_single_kernel<K, PRIMAL, DIM, TGV, ADJ_MAXR>), dim3((unsigned)blocks),
                        dim3(ADJ_THREADS), lds, st, P, tiles_x, ntiles);
 }
 
