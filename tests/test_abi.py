@@ -79,6 +79,20 @@ def test_pack_halos_rejects_what_a_slab_does_not_send():
     assert lib.gr_tgv_pack_halos(None, S, dual, None, None, None) == _lib.GR_ERR_INVALID
 
 
+def test_header_is_plain_c():
+    """include/graptor_amd.h compiles as C99 (no torch or C++ types at the
+    boundary), as a cgo or ctypes binding would read it."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    r = subprocess.run([gcc, "-fsyntax-only", "-std=c99", "-Wall", "-Werror", "-x", "c",
+                        os.path.join(ROOT, "include", "graptor_amd.h")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_param_struct_layout_matches_header():
     # 8 ints then 7 floats, no padding
     assert ctypes.sizeof(_lib.TgvParams) == 15 * 4
