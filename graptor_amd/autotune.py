@@ -18,11 +18,21 @@ FWD_BANDS = (4, 8, 12, 16, 24, 32)
 ADJ_TILES = (8, 16, 32)
 
 
+# the forward kernel's LDS limit per work-group (bytes) and its line pitch
+# (max(nx, ny) plus 4 zero candidates on each side, rounded up to odd)
+LDS_LIMIT = 160 * 1024
+
+
+def _slice_group(nslices):
+    # slice groups of the projector kernels: 3 slices take a padded group of 4
+    return 4 if nslices >= 3 else (2 if nslices == 2 else 1)
+
+
 def _lds_ok(problem, band):
-    K = 4 if problem.C * problem.Nz >= 4 else (2 if problem.C * problem.Nz >= 2 else 1)
+    K = _slice_group(problem.C * problem.Nz)
     pitch = max(problem.nx, problem.ny) + 8
     pitch |= 1
-    return band * pitch * K * 4 <= 96 * 1024
+    return band * pitch * K * 4 <= LDS_LIMIT
 
 
 def signature(problems):
@@ -55,7 +65,9 @@ def autotune(problems, iters=3, plan=None, cache=None):
             g.set_tuning(fwd_band=band, adj_tile_y=16)
         splan.retune()
         fwd_ms[band] = splan.profile(iters)[1] / iters
-    best_band = min(fwd_ms, key=fwd_ms.get)
+    # no candidate fits (very wide images): the forward band stays as it is
+    # (0 = the kernel's automatic choice)
+    best_band = min(fwd_ms, key=fwd_ms.get) if fwd_ms else 0
     for tile in ADJ_TILES:
         for g in geoms.values():
             g.set_tuning(fwd_band=best_band, adj_tile_y=tile)

@@ -100,6 +100,11 @@ bool valid_halos(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_stat
     return true;
 }
 
+bool has_halos(const gr_tgv_state* s)
+{
+    return s->halo_ubar_hi || s->halo_pbar_lo || s->halo_v_lo || s->halo_w_hi;
+}
+
 bool valid_state(const gr_geometry* g, const gr_tgv_params* p, const gr_tgv_state* s,
                  bool need_ku)
 {
@@ -421,7 +426,7 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
                         plp.items.push_back(DualItem{i, x0, y0, sl});
         }
         // dual items: 64 x 4 tiles of every plane
-        const int maxc = P.nch <= 4 ? 4 : 8;
+        const int maxc = P.nch <= 4 ? 4 : P.nch <= 8 ? 8 : P.nch;  // > 8 rejected at launch
         auto dk = std::make_tuple(P.dim, P.coupling, P.tgv, maxc);
         if (!dual_idx.count(dk)) {
             dual_idx[dk] = pl->dual.size();
@@ -730,6 +735,13 @@ int gr_tgv_iterate(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_s
         set_error("tgv_iterate: invalid parameters or missing state buffer");
         return GR_ERR_INVALID;
     }
+    if (has_halos(s)) {
+        // a slab's halo planes must be refreshed between the dual and the
+        // primal step of every iteration, which only the caller can do
+        set_error("tgv_iterate: z halo planes need an exchange inside every iteration; "
+                  "run slabs through gr_tgv_dual_step / gr_tgv_primal_step");
+        return GR_ERR_UNSUPPORTED;
+    }
     if (n_iter == 0) return GR_OK;
     const ProbDev P = make_prob(gg, p, s);
     return iterate_single(gg, P, n_iter, ku_on_last, stream_of(stream), nullptr, s);
@@ -766,6 +778,11 @@ int gr_tgv_profile(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_s
     if (!valid_params(gg, p) || n_iter <= 0 || !ms_out || !valid_state(gg, p, s, false)) {
         set_error("tgv_profile: invalid parameters or missing state buffer");
         return GR_ERR_INVALID;
+    }
+    if (has_halos(s)) {
+        set_error("tgv_profile: z halo planes need an exchange inside every iteration; "
+                  "run slabs through gr_tgv_dual_step / gr_tgv_primal_step");
+        return GR_ERR_UNSUPPORTED;
     }
     const ProbDev P = make_prob(gg, p, s);
     return iterate_single(gg, P, n_iter, 0, stream_of(stream), ms_out, s);
@@ -817,8 +834,14 @@ int gr_plan_create(int nprob, const gr_geometry* const* geoms, const gr_tgv_para
             set_error("plan_create: problem %d has invalid parameters or state", i);
             return GR_ERR_INVALID;
         }
-        if (states[i].halo_ubar_hi || states[i].halo_v_lo) {
+        if (has_halos(&states[i])) {
             set_error("plan_create: problem %d has z halo planes (slabs run singly)", i);
+            return GR_ERR_UNSUPPORTED;
+        }
+        if (prms[i].nch > 8) {
+            // the batched dual kernel holds at most 8 channels of a voxel
+            set_error("plan_create: problem %d has %d channels; batches hold at most 8", i,
+                      prms[i].nch);
             return GR_ERR_UNSUPPORTED;
         }
     }

@@ -21,7 +21,7 @@ import time
 import numpy as np
 
 from . import io as gio
-from ._lib import GraptorMemoryError
+from ._lib import GraptorError, GraptorMemoryError
 
 # --------------------------------------------------------------------------
 # pre-processing (H5:56-313)
@@ -170,6 +170,54 @@ def slab_schedule(n, possible, overlay):
     return out
 
 
+def _device_oom_types():
+    try:
+        import torch
+        return (GraptorMemoryError, torch.cuda.OutOfMemoryError)
+    except ImportError:  # pragma: no cover - torch is a dependency of the solver
+        return (GraptorMemoryError,)
+
+
+def solve_slabs(solve, nz, possible, overlay, C, nd, na, log=None):
+    """The slab loop of RC:405-536 around a solver callable.
+
+    ``solve(start, end, ov_s, ov_e)`` solves slices start..end (inclusive) and
+    returns (u (C, nd, nd, n), ku (C, na, n, nd)); the interior of each slab
+    (its overlap slices excluded) is stored.  A device out-of-memory error
+    (GraptorMemoryError / torch OOM, the reference's cl.MemoryError) and any
+    other library failure (GraptorError, its cl.Error) shrink the slab to
+    int(0.7 * possible) slices and restart the schedule (RC:510-536); below
+    one slice the memory case raises MemoryError and the other case
+    GraptorError.  Returns (U (C, nd, nd, nz), sinogram (C, na, nz, nd), the
+    slab size that succeeded)."""
+    log = sys.stderr if log is None else log
+    oom = _device_oom_types()
+    U = np.zeros((C, nd, nd, nz))
+    sino_out = np.zeros((C, na, nz, nd))
+    while True:
+        try:
+            for start, end, ov_s, ov_e in slab_schedule(nz, possible, overlay):
+                u, ku = solve(start, end, ov_s, ov_e)
+                n = end + 1 - start
+                U[:, :, :, start + ov_s:end + 1 - ov_e] = u[:, :, :, ov_s:n - ov_e]
+                sino_out[:, :, start + ov_s:end + 1 - ov_e, :] = ku[:, :, ov_s:n - ov_e, :]
+            return U, sino_out, possible
+        except oom as err:
+            possible = int(possible * 0.7)
+            print("Warning: Computation requires too much memory, the problem is split, "
+                  f"trying to compute {possible} slices at once.", file=log)
+            if possible < 1:
+                raise MemoryError("Even a single slice requires too much memory.") from err
+        except GraptorError as err:
+            possible = int(possible * 0.7)
+            print(f"An exception of type {type(err).__name__} occurred. Arguments:\n{err.args!r}",
+                  file=log)
+            print("Warning: unexpected device error occurred, trying to restart with "
+                  f"{possible} slices at once.", file=log)
+            if possible < 1:
+                raise GraptorError("Even a single slice creates an unknown error.") from err
+
+
 def faktor_rescale(solution, sinogram, ratio, na):
     """Divide each channel of the solution and of the model sinogram by
     Faktor / na with Faktor = sum(sinogram_c) / ratio_c (RC:477-480)."""
@@ -244,29 +292,17 @@ def coupled_reconstruction_3d(names, result, mu, alpha, maxiter, method, sinogra
     possible = min(nz, max(1, slices_per_device(nd, na, C, free)))
     if chunksize >= 1:
         possible = min(possible, int(chunksize))
-    U = np.zeros((C, nd, nd, nz))
-    sino_out = np.zeros((C, na, nz, nd))
     t0 = time.time()
-    while True:
-        try:
-            for start, end, ov_s, ov_e in slab_schedule(nz, possible, overlay):
-                cur = [section[i] for i in range(start, end + 1)]
-                info = [start / float(nz), (end + 1 - start - ov_e) / float(nz), cur]
-                sc = [s[:, start:end + 1, :] for s in sinos]
-                u, ku = r3d.Reconstructionapproach3d(sc, angles + np.pi / 2, tuple(alpha), mu,
-                                                    maxiter, dev, [0, 0], sinogram_level[2],
-                                                    discrepancy, method, regularisation, info,
-                                                    L=L)
-                n = end + 1 - start
-                U[:, :, :, start + ov_s:end + 1 - ov_e] = u[:, :, :, ov_s:n - ov_e]
-                sino_out[:, :, start + ov_s:end + 1 - ov_e, :] = ku[:, :, ov_s:n - ov_e, :]
-            break
-        except (GraptorMemoryError, torch.cuda.OutOfMemoryError) as err:
-            possible = int(possible * 0.7)
-            print("Warning: Computation requires too much memory, the problem is split, "
-                  f"trying to compute {possible} slices at once.", file=sys.stderr)
-            if possible < 1:
-                raise MemoryError("Even a single slice requires too much memory.") from err
+
+    def solve(start, end, ov_s, ov_e):
+        cur = [section[i] for i in range(start, end + 1)]
+        info = [start / float(nz), (end + 1 - start - ov_e) / float(nz), cur]
+        sc = [s[:, start:end + 1, :] for s in sinos]
+        return r3d.Reconstructionapproach3d(sc, angles + np.pi / 2, tuple(alpha), mu, maxiter,
+                                            dev, [0, 0], sinogram_level[2], discrepancy, method,
+                                            regularisation, info, L=L)
+
+    U, sino_out, _ = solve_slabs(solve, nz, possible, overlay, C, nd, na)
     print("\nTime required for reconstruction:", int(10.0 * (time.time() - t0)) / 10.0,
           " seconds.\n")
     solution = np.ascontiguousarray(U.transpose(0, 3, 1, 2))      # (C, Nz, Nx, Ny)

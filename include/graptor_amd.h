@@ -135,7 +135,9 @@ typedef struct gr_tgv_state {
  * (inside the backprojection kernel when the geometry's fusion is on, in
  * which case st->ktl may be NULL).  With ku_on_last,
  * the last iteration also stores R(ubar) into ku, the sinogram the reference
- * returns (2D:840-841). */
+ * returns (2D:840-841).  A state with z halo planes is rejected with
+ * GR_ERR_UNSUPPORTED (its halos must be exchanged inside every iteration:
+ * slabs run through gr_tgv_dual_step / gr_tgv_primal_step). */
 int gr_tgv_iterate(const gr_geometry *g, const gr_tgv_params *prm,
                    const gr_tgv_state *st, int n_iter, int ku_on_last,
                    void *stream);
@@ -191,7 +193,8 @@ int gr_tgv_energy(const gr_geometry *g, const gr_tgv_params *prm,
  * coupling, discrepancy and state, advanced together with one launch per
  * kernel class and phase (at most a few launches per phase for the whole
  * batch).  The plan keeps device copies of the descriptors; the state
- * buffers stay caller-owned and must outlive the plan. */
+ * buffers stay caller-owned and must outlive the plan.  A problem with more
+ * than 8 channels or with z halo planes gives GR_ERR_UNSUPPORTED. */
 typedef struct gr_plan gr_plan;
 int gr_plan_create(int nprob, const gr_geometry *const *geoms,
                    const gr_tgv_params *prms, const gr_tgv_state *states,

@@ -20,7 +20,10 @@ SPECS = [
     (2, 72, 33, 2, 1, "frobenius", "L2"),
     (2, 48, 60, 4, 1, "nuclear", "KL"),
     (3, 36, 20, 2, 5, "frobenius", "KL"),
+    (2, 44, 28, 6, 1, "frobenius", "KL"),       # 5..8 channels: the 8-channel kernels
+    (3, 32, 18, 8, 3, "uncorrelated", "L2"),
 ]
+MU = [1.0, 0.3, 0.2, 0.1, 0.5, 0.4, 0.25, 0.15, 0.35]
 
 
 def _make(seed, spec):
@@ -30,7 +33,7 @@ def _make(seed, spec):
     _, f, ofs = make_problem(seed, C, Nz, N, na, poisson=20.0)
     g = RadonGeometry((N, N), angles_deg(-70, 70, na), n_detectors=N)
     L = O.normest_power(ofs, N, N, N, iters=60, tol=1e-9)
-    mu = [1.0, 0.3, 0.2, 0.1][:C]
+    mu = MU[:C]
     prob = TgvProblem(f.astype(np.float32), g, (4.0, 1.0), mu, dim=dim, coupling=cpl,
                       discrepancy=disc, L=L)
     return prob, f.astype(np.float32), ofs, L, mu
@@ -50,12 +53,25 @@ def test_batch_matches_individual_solves():
         ek = rel(b[0].ku.cpu().numpy(), s[0].ku.cpu().numpy())
         print(f"problem {i} {SPECS[i]}: batch vs single u {eu:.1e} ku {ek:.1e}", flush=True)
         assert eu <= 1e-6 and ek <= 1e-6
-    # and two of them against the FP32 restatement of the reference kernels
-    for i in (1, 4):
-        prob, f, ofs, L, mu = batch[i]
+    # and every one of them against the FP32 restatement of the reference kernels
+    for i, (prob, f, ofs, L, mu) in enumerate(batch):
         dim, N, na, C, Nz, cpl, disc = SPECS[i]
         r = cref.solve(f, ofs, (4.0, 1.0), mu, 40, L, cpl, disc, dim=dim)
-        assert rel(prob.u.cpu().numpy(), r["u"]) <= 1e-4
+        eu = rel(prob.u.cpu().numpy(), r["u"])
+        ek = rel(prob.ku.cpu().numpy(), r["ku"])
+        print(f"problem {i} {SPECS[i]}: batch vs FP32 restatement u {eu:.1e} ku {ek:.1e}",
+              flush=True)
+        assert eu <= 1e-4 and ek <= 1e-4
+
+
+def test_plan_rejects_more_than_eight_channels():
+    """The batched dual kernel holds at most 8 channels of a voxel: a plan
+    with a 9-channel problem is refused instead of dropping channels."""
+    from graptor_amd.batch import BatchPlan
+    ok = _make(480, SPECS[0])[0]
+    nine = _make(481, (2, 32, 16, 9, 1, "frobenius", "KL"))[0]
+    with pytest.raises(NotImplementedError, match="at most 8"):
+        BatchPlan([ok, nine])
 
 
 def test_graph_replay_matches_eager():
@@ -72,6 +88,47 @@ def test_graph_replay_matches_eager():
         for i, (a, b) in enumerate(zip(eager, graph)):
             assert rel(b.u.cpu().numpy(), a.u.cpu().numpy()) <= 1e-6, (rep, i)
             assert rel(b.lam.cpu().numpy(), a.lam.cpu().numpy()) <= 1e-6, (rep, i)
+    # the replayed iterations against the FP32 restatement, every problem
+    for i, s in enumerate(SPECS):
+        _, f, ofs, L, mu = _make(700 + i, s)
+        dim, N, na, C, Nz, cpl, disc = s
+        r = cref.solve(f, ofs, (4.0, 1.0), mu, 36, L, cpl, disc, dim=dim)
+        err = rel(graph[i].u.cpu().numpy(), r["u"])
+        print(f"problem {i} {s}: graph replay vs FP32 restatement u {err:.1e}", flush=True)
+        assert err <= 1e-4, i
+
+
+def test_batched_2d_shape_against_restatement():
+    """A seeded subset of the 64-problem batch (N in 96..384, 30..180 angles
+    over [-70, 70) degrees, 1..4 channels, Poisson data), with tiles from the
+    autotuner, advanced through the captured graph, against the FP32
+    restatement problem by problem."""
+    from graptor_amd.autotune import autotune
+    from graptor_amd.batch import BatchPlan
+    from graptor_amd.phantoms import batched_2d_problem_set
+    from graptor_amd.solver import TgvProblem
+    dev = torch.device("cuda", 0)
+    specs = batched_2d_problem_set(n=8, device=dev)
+    probs = []
+    for sp in specs:
+        cpl = "frobenius" if sp["C"] > 1 else "uncorrelated"
+        mu = ([0.02] + [0.5] * 3)[:sp["C"]]
+        probs.append(TgvProblem(sp["f"], sp["geometry"], (4.0, 1.0), mu, dim=2, coupling=cpl,
+                                discrepancy="KL", normest_mode="power", device=dev, fused=False))
+    plan = BatchPlan(probs)
+    tune = autotune(probs, iters=2, plan=plan)
+    print(f"C3 subset: {[(sp['N'], sp['na'], sp['C']) for sp in specs]} tiles {tune['fwd_band']}"
+          f"/{tune['adj_tile_y']}", flush=True)
+    iters = 30
+    plan.iterate_graph(iters, block=10)
+    torch.cuda.synchronize()
+    for i, (sp, q) in enumerate(zip(specs, probs)):
+        ofs, _, _ = O.radon_struct((q.nx, q.ny), q.geometry.angles, n_detectors=q.nd)
+        r = cref.solve(q.f.cpu().numpy(), ofs, (4.0, 1.0), q.mu, iters, q.k.L, q.coupling, "KL",
+                       dim=2)
+        err = rel(q.u.cpu().numpy(), r["u"])
+        print(f"problem {i} N={sp['N']} na={sp['na']} C={sp['C']}: rel L2 {err:.1e}", flush=True)
+        assert err <= 1e-4, i
 
 
 def test_autotune_keeps_results():

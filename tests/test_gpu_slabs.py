@@ -2,6 +2,8 @@
 planes exchanged between the dual and the primal steps must give the same
 bits as the same volume solved whole, for uneven slab sizes, every coupling,
 TV and TGV, and a z step size other than one."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -78,12 +80,22 @@ def test_halo_state_validation():
     with pytest.raises(ValueError):
         p.iterate(1)
     p.state.halo_pbar_lo = plane.data_ptr()
-    p.iterate(1)
+    # a complete halo side: the single steps take it, the whole iteration
+    # refuses it (the halos must be exchanged between its dual and primal step)
+    lib = _lib.load()
+    _lib.check(lib.gr_tgv_dual_step(g.handle, ctypes.byref(p.params), ctypes.byref(p.state),
+                                    p._stream()))
     torch.cuda.synchronize()
+    with pytest.raises(NotImplementedError):
+        p.iterate(1)
+    with pytest.raises(NotImplementedError):
+        p.profile(1)
     p.state.halo_w_hi = plane.data_ptr()  # w_hi without ubar_hi
     with pytest.raises(ValueError):
         p.iterate(1)
-    assert _lib is not None
+    with pytest.raises(ValueError):
+        _lib.check(lib.gr_tgv_dual_step(g.handle, ctypes.byref(p.params), ctypes.byref(p.state),
+                                        p._stream()))
 
 
 def _pack(prob, phase, below, above):

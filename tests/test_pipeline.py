@@ -169,3 +169,73 @@ def test_pipeline_3d_slabs_cover_the_volume(tmp_path):
     for ch in chans:
         assert os.path.isfile(str(tmp_path / f"one_{ch}.mrc"))
     assert gio.read_mrc(str(tmp_path / "one_all_channels.mrc")).shape == (2, 433, 4 * 433)
+
+
+class _FakeSolver:
+    """Stands in for Reconstructionapproach3d: each slice's solution carries
+    its own index, and slabs above ``limit`` slices fail with ``exc``."""
+
+    def __init__(self, C, nd, na, limit, exc):
+        self.C, self.nd, self.na, self.limit, self.exc = C, nd, na, limit, exc
+        self.calls = []
+
+    def __call__(self, start, end, ov_s, ov_e):
+        n = end + 1 - start
+        self.calls.append((start, end, ov_s, ov_e))
+        if n > self.limit:
+            raise self.exc("fake device failure")
+        z = np.arange(start, end + 1, dtype=np.float64)
+        c = 1000.0 * np.arange(self.C)[:, None]
+        u = np.broadcast_to((z[None, :] + c)[:, None, None, :], (self.C, self.nd, self.nd, n))
+        ku = np.broadcast_to((z[None, :] + c)[:, None, :, None], (self.C, self.na, n, self.nd))
+        return u.copy(), ku.copy()
+
+
+@pytest.mark.parametrize("exc_name", ["GraptorMemoryError", "GraptorError", "OutOfMemoryError"])
+def test_slab_loop_shrinks_and_stores_every_slice(exc_name):
+    """RC:510-536: a device out-of-memory error and a generic device error
+    both shrink the slab to int(0.7 * possible) and restart the schedule;
+    every slice is stored once, from a slab that holds it as interior."""
+    import io
+
+    import torch
+
+    from graptor_amd import _lib
+    exc = torch.cuda.OutOfMemoryError if exc_name == "OutOfMemoryError" else getattr(_lib,
+                                                                                        exc_name)
+    C, nd, na, nz = 2, 3, 4, 23
+    fake = _FakeSolver(C, nd, na, limit=5, exc=exc)
+    log = io.StringIO()
+    U, S, possible = pl.solve_slabs(fake, nz, 10, 1, C, nd, na, log=log)
+    assert possible == 4  # 10 -> 7 -> 4 (int(0.7 * 7)); 4 <= 5 succeeds
+    want = np.arange(nz)[None, :] + 1000.0 * np.arange(C)[:, None]
+    assert np.array_equal(U, np.broadcast_to(want[:, None, None, :], U.shape))
+    assert np.array_equal(S, np.broadcast_to(want[:, None, :, None], S.shape))
+    # the failing attempts stop at their first slab; the last attempt runs the
+    # schedule of 4 slices with overlap 1
+    sizes = [e - s + 1 for s, e, _, _ in fake.calls]
+    assert sizes[:2] == [10, 7]
+    assert fake.calls[2:] == pl.slab_schedule(nz, 4, 1)
+    text = log.getvalue()
+    assert text.count("trying to") == 2
+    if exc_name == "GraptorError":
+        assert "unexpected device error" in text and "An exception of type GraptorError" in text
+    else:
+        assert "requires too much memory" in text
+
+
+@pytest.mark.parametrize("exc_name,raised,msg", [
+    ("GraptorMemoryError", MemoryError, "single slice requires too much memory"),
+    ("GraptorError", "GraptorError", "single slice creates an unknown error"),
+])
+def test_slab_loop_gives_up_below_one_slice(exc_name, raised, msg):
+    import io
+
+    from graptor_amd import _lib
+    exc = getattr(_lib, exc_name)
+    raised = getattr(_lib, raised) if isinstance(raised, str) else raised
+    fake = _FakeSolver(1, 2, 2, limit=0, exc=exc)
+    with pytest.raises(raised, match=msg):
+        pl.solve_slabs(fake, 6, 3, 0, 1, 2, 2, log=io.StringIO())
+    # 3 -> 2 -> 1 -> 0: three attempts, each failing on its first slab
+    assert [e - s + 1 for s, e, _, _ in fake.calls] == [3, 2, 1]
