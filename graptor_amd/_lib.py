@@ -35,8 +35,9 @@ EXPORTS = (
     "gr_cnn_conv_fwd", "gr_cnn_conv_wgrad", "gr_cnn_bias_grad", "gr_cnn_weight_flip",
     "gr_cnn_relu_bwd", "gr_cnn_pool", "gr_cnn_upsample", "gr_cnn_pack_bf16", "gr_cnn_mse_grad",
     "gr_cnn_adam", "gr_tgv_dual_step", "gr_tgv_primal_step", "gr_csr_apply", "gr_prox",
-    "gr_sumsq_f64", "gr_tgv_pack_halos",
+    "gr_sumsq_f64", "gr_tgv_pack_halos", "gr_tgv_energy_noalloc",
 )
+ENERGY_NOALLOC_MAX_CHANNELS = 32
 
 
 class GraptorError(RuntimeError):
@@ -103,6 +104,8 @@ def load(path: str | None = None):
                                 I, I, V], I),
             "gr_tgv_energy": ([V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState),
                                V, D, D, D, V, V], I),
+            "gr_tgv_energy_noalloc": ([V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState),
+                                       V, D, D, D, V, V], I),
             "gr_tgv_profile": ([V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState),
                                 I, V, V], I),
             "gr_plan_create": ([I, V, ctypes.POINTER(TgvParams), ctypes.POINTER(TgvState),

@@ -821,6 +821,40 @@ int gr_tgv_energy(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_st
     return GR_OK;
 }
 
+int gr_tgv_energy_noalloc(const gr_geometry* gg, const gr_tgv_params* p, const gr_tgv_state* s,
+                          const double* mu, double alpha0, double alpha1, double L, double* out,
+                          void* stream)
+{
+    if (!valid_params(gg, p) || !s || !s->u || !s->ku || !s->f || !mu || !out || L <= 0) {
+        set_error("tgv_energy_noalloc: invalid arguments");
+        return GR_ERR_INVALID;
+    }
+    if (p->regularisation == GR_REG_TGV && !s->p) {
+        set_error("tgv_energy_noalloc: p required for TGV");
+        return GR_ERR_INVALID;
+    }
+    if (p->nch > GR_ENERGY_NOALLOC_MAX_CHANNELS) {
+        set_error("tgv_energy_noalloc: %d channels exceed the %d its fixed scratch holds",
+                  p->nch, GR_ENERGY_NOALLOC_MAX_CHANNELS);
+        return GR_ERR_INVALID;
+    }
+    hipStream_t st = stream_of(stream);
+    ProbDev P = make_prob(gg, p, s);
+    ProbDev Pf = P;
+    Pf.ubar = s->u;  // project u itself into ku
+    int rc = launch_fwd(Pf, gg->g, p->nz * p->nch, FWD_WRITE, st);
+    if (rc) return rc;
+    const double scale = p->discrepancy == GR_DISC_KL ? 1.0 / L : 1.0;
+    const size_t per_ch = (size_t)gg->g.nd * gg->g.na * p->nz;
+    double data = 0.0, reg = 0.0;
+    rc = launch_energy_noalloc(P, mu, alpha0 * scale, alpha1 * scale, per_ch, &data, &reg, st);
+    if (rc) return rc;
+    out[0] = data;
+    out[1] = reg;
+    out[2] = data + reg;
+    return GR_OK;
+}
+
 int gr_plan_create(int nprob, const gr_geometry* const* geoms, const gr_tgv_params* prms,
                    const gr_tgv_state* states, gr_plan** out)
 {

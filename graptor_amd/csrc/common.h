@@ -180,6 +180,8 @@ int launch_normest(const Geometry& g, const double* x0_host, int iters, int mode
                    double tol, double* out, hipStream_t st);
 int launch_feasible(const Geometry& g, float* f, int nslices, hipStream_t st);
 int launch_energy(const ProbDev& P, double a0, double a1, double* out_reg, hipStream_t st);
+int launch_energy_noalloc(const ProbDev& P, const double* mu, double a0, double a1,
+                          size_t per_ch, double* out_data, double* out_reg, hipStream_t st);
 int launch_data_energy(const float* ku, const float* f, const double* mu, int nch,
                        size_t per_ch, int disc, double* out, hipStream_t st);
 

@@ -11,6 +11,7 @@
 #include "stencil.h"
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace gr {
@@ -23,33 +24,41 @@ constexpr float W_OFF = 1.4142f;  // the reference's float literal for sqrt(2)
 
 // 2x2 Gram route of update_NormV_Nucl (2D:124-204): M = sum_k S_k e_k e_k^T
 // with S_k = min(1, 1/(alphainv sigma_k)); G treated as diagonal when its
-// off-diagonal is negligible against the smaller diagonal entry.
-__device__ __forceinline__ void nucl2d(float a, float b, float c, float ainv, float& m00,
+// off-diagonal is negligible against the smaller diagonal entry.  The route's
+// branches and formulas are the reference's, evaluated in FP64 from the FP32
+// Gram entries: in FP32 its sigma_2 = sqrt((d - f)/2) and second eigenvector
+// (h a + b^2, h b + b c), h = (e - f)/2, cancel catastrophically when the
+// Gram matrix is near rank one, which moves a whole FP32 iteration by 1e-2
+// relative after 150 steps (two FP32 compilations of the reference text differ
+// from each other by as much); in FP64 the route equals the exact spectral
+// projection to 1e-7 and the iteration stays within 1e-6 of FP64 throughout.
+__device__ __forceinline__ void nucl2d(float af, float bf, float cf, float ainvf, float& m00,
                                        float& m01, float& m11)
 {
-    const float eps = 1e-5f;
-    const float o = fminf(a, c);
-    if ((b * b) / (o * o) > eps && o * o > 0.f) {
-        const float d = a + c, e = a - c;
-        float f = 4.f * b * b + e * e;
-        f = f < 0.f ? 0.f : sqrtf(f);
-        const float s2 = d - f > 0.f ? sqrtf((d - f) * 0.5f) : 0.f;
-        const float s1 = d + f > 0.f ? sqrtf((d + f) * 0.5f) : 0.f;
-        const float g = (e + f) * 0.5f, hh = (e - f) * 0.5f;
-        float v0 = g * a + b * b, v1 = g * b + c * b;
-        float v2 = hh * a + b * b, v3 = hh * b + b * c;
-        const float n1 = sqrtf(v0 * v0 + v1 * v1), n2 = sqrtf(v2 * v2 + v3 * v3);
-        if (n1 > 0.f) { v0 /= n1; v1 /= n1; }
-        if (n2 > 0.f) { v2 /= n2; v3 /= n2; }
-        const float S1 = s1 * ainv > 1.f ? 1.f / (s1 * ainv) : 1.f;
-        const float S2 = s2 * ainv > 1.f ? 1.f / (s2 * ainv) : 1.f;
-        m00 = v0 * v0 * S1 + v2 * v2 * S2;
-        m11 = v1 * v1 * S1 + v3 * v3 * S2;
-        m01 = S1 * v0 * v1 + S2 * v2 * v3;
+    const double a = af, b = bf, c = cf, ainv = ainvf;
+    const double eps = 1e-5;
+    const double o = fmin(a, c);
+    if ((b * b) / (o * o) > eps && o * o > 0.0) {
+        const double d = a + c, e = a - c;
+        double f = 4.0 * b * b + e * e;
+        f = f < 0.0 ? 0.0 : sqrt(f);
+        const double s2 = d - f > 0.0 ? sqrt((d - f) * 0.5) : 0.0;
+        const double s1 = d + f > 0.0 ? sqrt((d + f) * 0.5) : 0.0;
+        const double g = (e + f) * 0.5, hh = (e - f) * 0.5;
+        double v0 = g * a + b * b, v1 = g * b + c * b;
+        double v2 = hh * a + b * b, v3 = hh * b + b * c;
+        const double n1 = sqrt(v0 * v0 + v1 * v1), n2 = sqrt(v2 * v2 + v3 * v3);
+        if (n1 > 0.0) { v0 /= n1; v1 /= n1; }
+        if (n2 > 0.0) { v2 /= n2; v3 /= n2; }
+        const double S1 = s1 * ainv > 1.0 ? 1.0 / (s1 * ainv) : 1.0;
+        const double S2 = s2 * ainv > 1.0 ? 1.0 / (s2 * ainv) : 1.0;
+        m00 = (float)(v0 * v0 * S1 + v2 * v2 * S2);
+        m11 = (float)(v1 * v1 * S1 + v3 * v3 * S2);
+        m01 = (float)(S1 * v0 * v1 + S2 * v2 * v3);
     } else {
-        const float s1 = sqrtf(a), s2 = sqrtf(c);
-        m00 = ainv * s1 > 1.f ? 1.f / (ainv * s1) : 1.f;
-        m11 = ainv * s2 > 1.f ? 1.f / (ainv * s2) : 1.f;
+        const double s1 = sqrt(a), s2 = sqrt(c);
+        m00 = (float)(ainv * s1 > 1.0 ? 1.0 / (ainv * s1) : 1.0);
+        m11 = (float)(ainv * s2 > 1.0 ? 1.0 / (ainv * s2) : 1.0);
         m01 = 0.f;
     }
 }
@@ -442,6 +451,8 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_batch_kernel(
 // ---- FP64 energy ----------------------------------------------------------
 
 constexpr int EN_THREADS = 256;
+constexpr int EN_REG_BLOCKS = 1024;   // block partials of the regulariser
+constexpr int EN_DATA_BLOCKS = 256;   // block partials per channel of the data term
 
 __device__ double block_sum(double v, double* red)
 {
@@ -946,14 +957,10 @@ static void reg_launch(const ProbDev& P, double a0, double a1, double* partial, 
                        P.u, P.p, P.nx, P.ny, P.nz, P.nch, (double)P.hinv, a0, a1, partial);
 }
 
-int launch_energy(const ProbDev& P, double a0, double a1, double* out_reg, hipStream_t st)
+// block partials of the regulariser: EN_REG_BLOCKS doubles
+static void reg_partials(const ProbDev& P, double a0, double a1, double* partial, hipStream_t st)
 {
-    const int nb = 1024;
-    double* partial = nullptr;
-    if (hipMalloc(&partial, sizeof(double) * nb) != hipSuccess) {
-        set_error("energy: device allocation failed");
-        return GR_ERR_OOM;
-    }
+    const int nb = EN_REG_BLOCKS;
 #define GR_REG(DM, CP)                                                   \
     do {                                                                 \
         if (P.tgv) reg_launch<DM, CP, true>(P, a0, a1, partial, nb, st); \
@@ -969,30 +976,63 @@ int launch_energy(const ProbDev& P, double a0, double a1, double* out_reg, hipSt
         else GR_REG(3, GR_COUPLING_NUCLEAR);
     }
 #undef GR_REG
-    std::vector<double> h(nb);
+}
+
+// block partials of the data term: EN_DATA_BLOCKS doubles per channel
+static void data_partials(const float* ku, const float* f, int nch, size_t per_ch, int disc,
+                          double* partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(data_energy_kernel, dim3(EN_DATA_BLOCKS, nch), dim3(EN_THREADS), 0, st, ku,
+                       f, per_ch, disc, partial);
+}
+
+// The host's final sums, shared by both entry points so that they agree bit
+// for bit: the regulariser's partials in block order; the data term as
+// sum_c mu_c (partials of channel c in block order).
+static double sum_reg(const double* h)
+{
+    double s = 0.0;
+    for (int b = 0; b < EN_REG_BLOCKS; ++b) s += h[b];
+    return s;
+}
+
+static double sum_data(const double* chan_sums, const double* mu, int nch)
+{
+    double s = 0.0;
+    for (int c = 0; c < nch; ++c) s += mu[c] * chan_sums[c];
+    return s;
+}
+
+int launch_energy(const ProbDev& P, double a0, double a1, double* out_reg, hipStream_t st)
+{
+    double* partial = nullptr;
+    if (hipMalloc(&partial, sizeof(double) * EN_REG_BLOCKS) != hipSuccess) {
+        set_error("energy: device allocation failed");
+        return GR_ERR_OOM;
+    }
+    reg_partials(P, a0, a1, partial, st);
+    std::vector<double> h(EN_REG_BLOCKS);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
-        e = hipMemcpyAsync(h.data(), partial, sizeof(double) * nb, hipMemcpyDeviceToHost, st);
+        e = hipMemcpyAsync(h.data(), partial, sizeof(double) * EN_REG_BLOCKS,
+                           hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(partial);
     if (e != hipSuccess) return hip_status(e, "energy");
-    double s = 0.0;
-    for (double x : h) s += x;
-    *out_reg = s;
+    *out_reg = sum_reg(h.data());
     return GR_OK;
 }
 
 int launch_data_energy(const float* ku, const float* f, const double* mu, int nch,
                        size_t per_ch, int disc, double* out, hipStream_t st)
 {
-    const int nb = 256;
+    const int nb = EN_DATA_BLOCKS;
     double* partial = nullptr;
     if (hipMalloc(&partial, sizeof(double) * nb * nch) != hipSuccess) {
         set_error("energy: device allocation failed");
         return GR_ERR_OOM;
     }
-    hipLaunchKernelGGL(data_energy_kernel, dim3(nb, nch), dim3(EN_THREADS), 0, st, ku, f, per_ch,
-                       disc, partial);
+    data_partials(ku, f, nch, per_ch, disc, partial, st);
     std::vector<double> h((size_t)nb * nch);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
@@ -1000,13 +1040,117 @@ int launch_data_energy(const float* ku, const float* f, const double* mu, int nc
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(partial);
     if (e != hipSuccess) return hip_status(e, "data energy");
-    double s = 0.0;
-    for (int c = 0; c < nch; ++c) {
-        double sc = 0.0;
-        for (int b = 0; b < nb; ++b) sc += h[(size_t)c * nb + b];
-        s += mu[c] * sc;
+    std::vector<double> sc(nch, 0.0);
+    for (int c = 0; c < nch; ++c)
+        for (int b = 0; b < nb; ++b) sc[c] += h[(size_t)c * nb + b];
+    *out = sum_data(sc.data(), mu, nch);
+    return GR_OK;
+}
+
+// ---- allocation-free energy -------------------------------------------------
+//
+// The block partials and their per-channel sums live in `scratch`, a fixed
+// array of this library on every device it runs on (held until the process
+// ends).  Its host-side name is no device address: the address is taken once
+// per device with hipGetSymbolAddress, with that device current, and only that
+// pointer is handed to kernels and copies.  Every slot a call reads is written
+// earlier in the same call (each block writes its partial, one thread per sum
+// writes a sum), so nothing needs clearing; calls that overlap on one device
+// share the array and race.
+namespace {
+
+constexpr size_t EN_NOALLOC_REG = 0;                                      // EN_REG_BLOCKS partials
+constexpr size_t EN_NOALLOC_DATA = EN_NOALLOC_REG + EN_REG_BLOCKS;        // nb per channel
+constexpr size_t EN_NOALLOC_SUMS =
+    EN_NOALLOC_DATA + (size_t)EN_DATA_BLOCKS * GR_ENERGY_NOALLOC_MAX_CHANNELS;  // 1 + nch sums
+constexpr size_t EN_NOALLOC_DOUBLES = EN_NOALLOC_SUMS + 1 + GR_ENERGY_NOALLOC_MAX_CHANNELS;
+
+}  // namespace
+
+// 9,249 doubles (72.3 KiB) per device
+static __device__ __align__(256) double scratch[EN_NOALLOC_DOUBLES];
+
+namespace {
+
+// sums[0] = regulariser partials in block order; sums[1 + c] = data partials
+// of channel c in block order (the order of the host sums above)
+__global__ __launch_bounds__(64) void energy_sums_kernel(const double* __restrict__ reg,
+                                                         const double* __restrict__ data, int nb,
+                                                         int nch, double* __restrict__ sums)
+{
+    const int t = threadIdx.x;
+    if (t == 0) {
+        double s = 0.0;
+        for (int b = 0; b < EN_REG_BLOCKS; ++b) s += reg[b];
+        sums[0] = s;
+    } else if (t <= nch) {
+        const double* d = data + (size_t)(t - 1) * nb;
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += d[b];
+        sums[t] = s;
     }
-    *out = s;
+}
+
+constexpr int EN_MAX_DEVICES = 64;
+
+double* noalloc_scratch(int* rc)
+{
+    static std::mutex mtx;
+    static double* ptr[EN_MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *rc = hip_status(e, "energy_noalloc: current device");
+        return nullptr;
+    }
+    if (dev < 0 || dev >= EN_MAX_DEVICES) {
+        set_error("energy_noalloc: device %d beyond the %d this library tracks", dev,
+                  EN_MAX_DEVICES);
+        *rc = GR_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mtx);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *rc = hip_status(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                             "energy_noalloc: address of the scratch array");
+            return nullptr;
+        }
+        ptr[dev] = static_cast<double*>(p);
+    }
+    return ptr[dev];
+}
+
+}  // namespace
+
+int launch_energy_noalloc(const ProbDev& P, const double* mu, double a0, double a1,
+                          size_t per_ch, double* out_data, double* out_reg, hipStream_t st)
+{
+    if (P.nch > GR_ENERGY_NOALLOC_MAX_CHANNELS) {
+        set_error("energy_noalloc: %d channels exceed the %d its fixed scratch holds", P.nch,
+                  GR_ENERGY_NOALLOC_MAX_CHANNELS);
+        return GR_ERR_INVALID;
+    }
+    int rc = GR_OK;
+    double* base = noalloc_scratch(&rc);
+    if (!base) return rc;
+    double* reg = base + EN_NOALLOC_REG;
+    double* data = base + EN_NOALLOC_DATA;
+    double* sums = base + EN_NOALLOC_SUMS;
+    data_partials(P.ku, P.f, P.nch, per_ch, P.disc, data, st);
+    reg_partials(P, a0, a1, reg, st);
+    hipLaunchKernelGGL(energy_sums_kernel, dim3(1), dim3(64), 0, st, reg, data, EN_DATA_BLOCKS,
+                       P.nch, sums);
+    double h[1 + GR_ENERGY_NOALLOC_MAX_CHANNELS];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h, sums, sizeof(double) * (1 + P.nch), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_status(e, "energy_noalloc");
+    *out_reg = h[0];
+    *out_data = sum_data(h + 1, mu, P.nch);
     return GR_OK;
 }
 

@@ -188,6 +188,19 @@ int gr_tgv_energy(const gr_geometry *g, const gr_tgv_params *prm,
                   const gr_tgv_state *st, const double *mu, double alpha0,
                   double alpha1, double L, double *out, void *stream);
 
+/* gr_tgv_energy without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as tight
+ * serving or convergence-monitoring loops and harnesses that forbid
+ * hipMalloc.  Same arguments, same results bit for bit; the FP64 partial sums
+ * live in a fixed array of the library on each device (72 KiB, held until the
+ * process ends), so at most GR_ENERGY_NOALLOC_MAX_CHANNELS channels are
+ * accepted (more: GR_ERR_INVALID).  Calls that overlap on one device share
+ * that array and race; serialise them.  Synchronises the stream. */
+enum { GR_ENERGY_NOALLOC_MAX_CHANNELS = 32 };
+int gr_tgv_energy_noalloc(const gr_geometry *g, const gr_tgv_params *prm,
+                          const gr_tgv_state *st, const double *mu, double alpha0,
+                          double alpha1, double L, double *out, void *stream);
+
 /* Batches (no counterpart in the reference, which solves one problem per
  * call): nprob independent problems with their own geometry, sizes,
  * coupling, discrepancy and state, advanced together with one launch per

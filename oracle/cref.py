@@ -12,6 +12,8 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "_build", "libcpuref.so")
+# the same source built with FMA contraction (a second FP32 compilation)
+LIB_FMA = os.path.join(_HERE, "_build", "libcpuref_fma.so")
 _lib = None
 
 CPL = {"uncorrelated": 0, "frobenius": 1, "nuclear": 2}

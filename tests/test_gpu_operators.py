@@ -1,0 +1,185 @@
+"""Per-operator GPU parity against the FP64 oracle at 1e-5 relative L2.
+
+From a random state (fields scaled so that the coupled projections are
+active at some voxels and inactive at others) one iteration of the product
+path (gr_tgv_iterate, two streams) must give, array by array, what the
+oracle's operators give composed in the reference's order (3D:1194-1221):
+
+    v1, w1  = dual step of (ubar0, pbar0, v0, w0)    update_v/_w + projections
+    lam1    = prox(lam0 + sigma' R ubar0)            radon + update_lambda_*
+    ktl1    = R^T lam1                               radon_ad
+    u1, p1  = primal step, ubar1/pbar1 extrapolated  update_u/_p + update_extra
+
+and each standalone entry point (gr_tgv_dual_step, gr_tgv_primal_step,
+gr_prox) must match its operator alone, for every coupling x 2D/3D x TV/TGV
+x KL/L2.  2D nuclear coupling follows the FP64 evaluation of the reference's
+closed-form route (test_oracle.py shows why its FP32 evaluation cannot be
+the bar)."""
+import ctypes
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import tgv_oracle as O
+from tests.helpers import angles_deg, make_problem, rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+COMBOS = list(itertools.product((2, 3), O.COUPLINGS, ("TGV", "TV"), ("KL", "L2")))
+IDS = ["-".join(map(str, c)) for c in COMBOS]
+BAR = 1e-5
+
+
+def _setup(dim, coupling, reg, disc, seed=90):
+    """A TgvProblem whose state holds random fields; alpha is chosen so that
+    the scaled radii alpha/L^2 are (0.8, 0.5) and the fields straddle them."""
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    C, Nz, N, na = 3, (1 if dim == 2 else 4), 40, 20
+    _, f, ofs = make_problem(seed, C, Nz, N, na, poisson=40.0)
+    g = RadonGeometry((N, N), angles_deg(-70, 70, na), n_detectors=N)
+    L = O.normest_power(ofs, N, N, N)
+    alpha = (0.8 * L * L, 0.5 * L * L)
+    mu = [1.0, 0.4, 0.2]
+    prob = TgvProblem(f.astype(np.float32), g, alpha, mu, dim=dim, coupling=coupling,
+                      discrepancy=disc, regularisation=reg, stepsize=1.5 if dim == 3 else 1.0,
+                      L=L)
+    rng = np.random.default_rng(seed + 1)
+    # radii 0.5 (v) and 0.8 (w): these scales put the updated fields' pointwise
+    # norms on both sides of them
+    scale = {"u": 1.0, "ubar": 0.3, "p": 0.3, "pbar": 0.1, "v": 0.15, "w": 0.15, "lam": 0.5}
+    for name, s in scale.items():
+        t = getattr(prob, name)
+        if t is None:
+            continue
+        a = rng.standard_normal(tuple(t.shape)) * s
+        if name == "u":
+            a = np.abs(a)
+        t.copy_(torch.as_tensor(a.astype(np.float32)))
+    torch.cuda.synchronize()
+    return prob, f.astype(np.float32), ofs
+
+
+def _host(prob):
+    return {n: (getattr(prob, n).cpu().numpy().astype(np.float64)
+                if getattr(prob, n) is not None else None)
+            for n in ("u", "ubar", "p", "pbar", "v", "w", "lam", "ktl", "f")}
+
+
+def _consts(prob):
+    """The FP32 constants the kernels receive, widened to FP64."""
+    P = prob.params
+    f32 = lambda x: float(np.float32(x))
+    return dict(sigma=f32(P.sigma), tau=f32(P.tau), sigma_p=f32(P.sigma_p),
+                norming=f32(P.norming), ainv0=f32(P.alpha_inv0), ainv1=f32(P.alpha_inv1),
+                h=f32(P.stepsize), mfac=prob.k.mfac.astype(np.float32).astype(np.float64))
+
+
+def _ref_dual(s, k, prob):
+    dim, cpl = prob.dim, prob.coupling
+    mode = "ref2d" if (cpl == "nuclear" and dim == 2) else "svd"
+    tgv = prob.regularisation == "TGV"
+    q = O.grad(s["ubar"], dim, k["h"]) - (s["pbar"] if tgv else 0.0)
+    v = O.proj_v(s["v"] + k["sigma"] * q, k["ainv1"], cpl, dim, mode)
+    w = None
+    if tgv:
+        w = O.proj_w(s["w"] + k["sigma"] * O.symgrad(s["pbar"], dim, k["h"]), k["ainv0"], cpl,
+                     dim)
+    return v, w
+
+
+def _ref_prox(lam, ku, f, k, disc):
+    if disc == "KL":
+        return O.prox_kl(lam, ku, f, k["sigma_p"], k["mfac"])
+    return O.prox_l2(lam, ku, f, k["sigma_p"], k["mfac"])
+
+
+def _ref_primal(u, p, v, w, ktl, k, prob):
+    dim = prob.dim
+    un = np.maximum(0.0, u + k["tau"] * (O.div(v, dim, k["h"]) - k["norming"] * ktl))
+    pn = None
+    if prob.regularisation == "TGV":
+        pn = p + k["tau"] * (v + O.div2(w, dim, k["h"]))
+    return un, pn
+
+
+def _check(name, got, want, tag):
+    err = rel(got, want)
+    print(f"  {tag} {name}: rel L2 {err:.1e}", flush=True)
+    assert err <= BAR, (tag, name, err)
+
+
+@pytest.mark.parametrize("combo", COMBOS, ids=IDS)
+def test_one_iteration_operator_by_operator(combo):
+    dim, cpl, reg, disc = combo
+    print(f"one iteration {combo}", flush=True)
+    prob, f, ofs = _setup(dim, cpl, reg, disc)
+    s = _host(prob)
+    k = _consts(prob)
+    prob.iterate(1)
+    torch.cuda.synchronize()
+    g = _host(prob)
+    v1, w1 = _ref_dual(s, k, prob)
+    _check("v", g["v"], v1, "dual")
+    if w1 is not None:
+        _check("w", g["w"], w1, "dual")
+    lam1 = _ref_prox(s["lam"], O.radon_fwd(s["ubar"], ofs, prob.nd), s["f"], k, disc)
+    _check("lam", g["lam"], lam1, "forward+prox")
+    ktl1 = O.radon_adj(lam1, ofs, prob.nx, prob.ny)
+    _check("ktl", g["ktl"], ktl1, "backprojection")
+    # the primal step from the GPU's own v1, w1 and K^T lam1 (each checked above)
+    un, pn = _ref_primal(s["u"], s["p"], g["v"], g["w"], g["ktl"], k, prob)
+    _check("u", g["u"], un, "primal")
+    _check("ubar", g["ubar"], 2 * un - s["u"], "primal")
+    if pn is not None:
+        _check("p", g["p"], pn, "primal")
+        _check("pbar", g["pbar"], 2 * pn - s["p"], "primal")
+
+
+@pytest.mark.parametrize("combo", COMBOS, ids=IDS)
+def test_step_entry_points(combo):
+    from graptor_amd import _lib
+    dim, cpl, reg, disc = combo
+    print(f"entry points {combo}", flush=True)
+    lib = _lib.load()
+    prob, f, ofs = _setup(dim, cpl, reg, disc, seed=95)
+    k = _consts(prob)
+    st = prob._stream()
+    # dual step alone
+    s = _host(prob)
+    _lib.check(lib.gr_tgv_dual_step(prob.geometry.handle, ctypes.byref(prob.params),
+                                    ctypes.byref(prob.state), st))
+    torch.cuda.synchronize()
+    g = _host(prob)
+    v1, w1 = _ref_dual(s, k, prob)
+    _check("v", g["v"], v1, "gr_tgv_dual_step")
+    if w1 is not None:
+        _check("w", g["w"], w1, "gr_tgv_dual_step")
+    # primal step alone, reading a random K^T lam
+    rng = np.random.default_rng(7)
+    prob.ktl.copy_(torch.as_tensor(rng.standard_normal(tuple(prob.ktl.shape)).astype(np.float32)))
+    torch.cuda.synchronize()
+    s = _host(prob)
+    _lib.check(lib.gr_tgv_primal_step(prob.geometry.handle, ctypes.byref(prob.params),
+                                      ctypes.byref(prob.state), st))
+    torch.cuda.synchronize()
+    g = _host(prob)
+    un, pn = _ref_primal(s["u"], s["p"], s["v"], s["w"], s["ktl"], k, prob)
+    _check("u", g["u"], un, "gr_tgv_primal_step")
+    _check("ubar", g["ubar"], 2 * un - s["u"], "gr_tgv_primal_step")
+    if pn is not None:
+        _check("p", g["p"], pn, "gr_tgv_primal_step")
+        _check("pbar", g["pbar"], 2 * pn - s["p"], "gr_tgv_primal_step")
+    # prox alone on a random K u
+    ku = torch.as_tensor(rng.uniform(0, 3, tuple(prob.ku.shape)).astype(np.float32),
+                         device=prob.device)
+    lam0 = prob.lam.cpu().numpy().astype(np.float64)
+    _lib.check(lib.gr_prox(_lib.DISCREPANCY[disc], prob.params.sigma_p, prob.mfac.data_ptr(),
+                           prob.C, prob.Nz * prob.na * prob.nd, ku.data_ptr(), prob.f.data_ptr(),
+                           prob.lam.data_ptr(), st))
+    torch.cuda.synchronize()
+    want = _ref_prox(lam0, ku.cpu().numpy().astype(np.float64), s["f"], k, disc)
+    _check("lam", prob.lam.cpu().numpy(), want, "gr_prox")

@@ -61,11 +61,20 @@ def test_iteration_parity(combo):
     e64 = rel(u, r64["u"])
     print(f"  rel L2 u: vs FP32 restatement {e32:.2e}, vs FP64 oracle {e64:.2e}; "
           f"ku vs FP32 {rel(ku, r32['ku']):.2e}", flush=True)
-    tol = 5e-3 if (coupling == "nuclear" and dim == 2) else 1e-4
-    assert e32 <= tol
-    assert rel(ku, r32["ku"]) <= tol
-    if not (coupling == "nuclear" and dim == 2):
-        assert e64 <= 1e-4
+    assert e64 <= 1e-4
+    if coupling == "nuclear" and dim == 2:
+        # the reference's FP32 closed form of the 2D nuclear projection is
+        # itself off its FP64 evaluation by 1e-2 after 150 steps (and two FP32
+        # compilations of it differ by as much); the GPU evaluates that route
+        # in FP64, so it is held to the FP64 evaluation above, and to the FP32
+        # restatement only within the restatement's own distance from it
+        d32 = rel(r32["u"], r64["u"])
+        print(f"  2D nuclear: FP32 restatement vs FP64 evaluation of its route {d32:.2e}",
+              flush=True)
+        assert e32 <= d32 + 1e-4
+    else:
+        assert e32 <= 1e-4
+        assert rel(ku, r32["ku"]) <= 1e-4
     # primal energy: GPU FP64 reduction vs oracle at the GPU iterate
     data, regv, total = prob.energy()
     p = prob.p.cpu().numpy() if prob.p is not None else np.zeros((dim,) + u.shape)

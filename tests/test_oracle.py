@@ -303,3 +303,36 @@ def test_cref_jacobi_eigen():
     ev, V = cref.jacobi3(np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1], G[0, 2], G[1, 2]]))
     np.testing.assert_allclose(np.sort(ev), np.linalg.eigvalsh(G), rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(G @ V, V * ev[None, :], atol=1e-4)
+
+
+def test_reference_2d_nuclear_fp32_route_is_ill_conditioned():
+    """The reference's closed-form 2D nuclear projection (2D:124-204) is exact
+    in FP64 (it equals the SVD projection) but cancels catastrophically in
+    FP32: a whole FP32 iteration drifts from its FP64 evaluation by > 1e-3
+    within 150 steps, and two FP32 compilations of the same text (multiply-adds
+    rounded separately or contracted into FMAs, as OpenCL and HIP compilers
+    do by default) drift apart by as much.  Frobenius coupling, the control,
+    stays at FP32 accuracy in all three comparisons."""
+    import os
+    _, f, ofs = make_problem(31, 3, 1, 40, 24, poisson=50.0)
+    f = f.astype(np.float32)
+    L = O.normest_power(ofs, 40, 40, 40)
+    cref.load()
+    if not os.path.exists(cref.LIB_FMA):
+        cref.build()
+    fma = cref.load(cref.LIB_FMA)
+    out = {}
+    for cpl in ("nuclear", "frobenius"):
+        kw = dict(alpha=(4.0, 1.0), mu=[1.0, 0.3, 0.1], maxiter=150, L=L, coupling=cpl,
+                  discrepancy="KL", dim=2)
+        plain = cref.solve(f, ofs, **kw)["u"]
+        contracted = cref.solve(f, ofs, lib=fma, **kw)["u"]
+        r64 = O.solve(f.astype(np.float64), ofs, nuclear_mode="ref2d", **kw)["u"]
+        out[cpl] = (rel(plain, contracted), rel(plain, r64), rel(contracted, r64))
+        if cpl == "nuclear":
+            s64 = O.solve(f.astype(np.float64), ofs, nuclear_mode="svd", **kw)["u"]
+            assert rel(r64, s64) < 1e-6
+    print(f"2D nuclear: FP32 builds apart {out['nuclear'][0]:.1e}, vs FP64 "
+          f"{out['nuclear'][1]:.1e} / {out['nuclear'][2]:.1e}; Frobenius {out['frobenius']}")
+    assert min(out["nuclear"]) > 1e-3
+    assert max(out["frobenius"]) < 2e-5
