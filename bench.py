@@ -26,6 +26,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# FP32 vector FMA peak without packed instructions (this build compiles with
+# packed FP32 off): 256 CUs x 64 FMA/clk x 2 flops x 2.4 GHz
+VALU_PEAK_TFLOPS = 78.6
 ALPHA = (4.0, 1.0)
 MU = (0.2, 5e-4, 4e-3, 5e-4)
 KERNELS = ("dual_single_kernel", "fwd_single_kernel", "adj_single_kernel",
@@ -65,22 +68,29 @@ def launch_counts(n, fused):
     return (plain + 1, n, n, plain + 1, n - plain - 1)
 
 
-def cpu_baseline(f_host, angles, L, iters):
-    """Time the FP32 C/OpenMP restatement on this host (bounded sample)."""
+def _native_cref():
+    """The FP32 C restatement built for this host's CPU (falls back to the
+    prebuilt x86-64-v3 library)."""
     from oracle import cref
-    from oracle import tgv_oracle as O
-    lib_path = None
-    try:  # native build for this host's CPU
+    try:
         tmp = tempfile.mkdtemp(prefix="cref_")
         lib_path = os.path.join(tmp, "libcpuref_native.so")
         subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-shared",
                         "-std=c11", os.path.join(ROOT, "oracle", "cpu_ref.c"), "-o", lib_path,
                         "-lm"], check=True, capture_output=True, timeout=120)
-        lib = cref.load(lib_path)
-        arch = "native"
+        return lib_path, "native"
     except Exception:
-        lib = cref.load()
-        arch = "x86-64-v3"
+        cref.load()
+        return cref._LIB, "x86-64-v3"
+
+
+def _time_cref(lib_path, f_host, angles, L, iters):
+    """Seconds per iteration of the C restatement: the difference of a
+    (1 + iters)- and a 1-iteration run divided by iters (allocation and
+    set-up excluded)."""
+    from oracle import cref
+    from oracle import tgv_oracle as O
+    lib = cref.load(lib_path)
     C, Nz, na, nd = f_host.shape
     ofs, _, _ = O.radon_struct((nd, nd), angles, n_detectors=nd)
     kw = dict(alpha=ALPHA, mu=MU, L=L, coupling="frobenius", discrepancy="KL", dim=3, lib=lib)
@@ -90,13 +100,51 @@ def cpu_baseline(f_host, angles, L, iters):
     cref.solve(f_host, ofs, maxiter=1 + iters, **kw)
     t2 = time.perf_counter()
     per_iter = ((t2 - t1) - (t1 - t0)) / iters
-    per_iter = per_iter if per_iter > 0 else (t2 - t1) / (1 + iters)
+    return per_iter if per_iter > 0 else (t2 - t1) / (1 + iters)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(f_host, angles, L, iters, single_iters=1):
+    """Time the FP32 C/OpenMP restatement on this host (bounded samples): on
+    all cores of this process, and on one thread in a child process."""
+    lib_path, arch = _native_cref()
+    per_iter = _time_cref(lib_path, f_host, angles, L, iters)
     threads = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    C, Nz, na, nd = f_host.shape
     vox = C * Nz * nd * nd
+    one = None
+    if single_iters > 0:
+        tmp = tempfile.mkdtemp(prefix="cref1_")
+        fpath = os.path.join(tmp, "f.npy")
+        import numpy as np
+        np.save(fpath, f_host)
+        np.save(os.path.join(tmp, "angles.npy"), np.asarray(angles, np.float64))
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        code = (f"import sys, numpy as np; sys.path.insert(0, {ROOT!r}); import bench; "
+                f"f = np.load({fpath!r}); a = np.load({os.path.join(tmp, 'angles.npy')!r}); "
+                f"print(bench._time_cref({lib_path!r}, f, a, {float(L)!r}, {int(single_iters)}))")
+        try:
+            out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                                 text=True, timeout=600, check=True)
+            one = vox / float(out.stdout.strip().splitlines()[-1])
+        except Exception as exc:  # the 1-thread figure is informative only
+            print(f"# 1-thread baseline failed: {exc}", file=sys.stderr, flush=True)
     return dict(value=vox / per_iter, unit="voxel-iters/s", cores=threads, kind="port",
-                sample=f"same {nd}x{nd}x{Nz}x{C} volume, {iters} iterations timed as the difference "
-                       f"of a {1 + iters}- and a 1-iteration run (allocation excluded), "
-                       f"gcc -O3 {arch} OpenMP",
+                cpu=_cpu_model(), value_1thread=one,
+                sample=f"same {nd}x{nd}x{Nz}x{C} volume, {iters} iterations on {threads} threads "
+                       f"and {single_iters} on 1 thread, each timed as the difference of an "
+                       f"(n+1)- and a 1-iteration run (allocation excluded), gcc -O3 {arch} "
+                       f"OpenMP, FP32 restatement of the reference kernels",
                 seconds_per_iter=per_iter)
 
 
@@ -529,6 +577,10 @@ def main():
     del f
     print(f"# problem ready: {prob.C}x{prob.Nz}x{prob.ny}x{prob.nx}, na={prob.na}, "
           f"L={prob.k.L:.6f}", file=sys.stderr, flush=True)
+    # per-kernel durations: HIP events around every launch of a profiled pass
+    # (one-stream order); run ahead of the warm-up so that the timed steps
+    # start at the clocks of a busy GPU
+    ms = prob.profile(args.profile_iters)
     prob.iterate(args.warmup)
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
@@ -546,8 +598,6 @@ def main():
     print(f"# timed {args.steps} iterations: wall {wall:.4f}s, events {gpu_ms / 1e3:.4f}s",
           file=sys.stderr, flush=True)
 
-    # per-kernel durations: HIP events around every launch of a profiled pass
-    ms = prob.profile(args.profile_iters)
     counts = launch_counts(args.profile_iters, prob.u2 is not None)
     nbytes = kernel_bytes(prob)
     kern = {k: dict(avg_ms=m / c, launches=c, bytes=nbytes[k],
@@ -616,11 +666,21 @@ def main():
                 counted[k] = (table[key]["fetch"] + table[key]["write"]) * 2 ** 20
     traffic = counted.get(dom)
     # the projectors, by the same algorithmic-bytes rule: bound by vector ALU
-    # and LDS work, so their byte rate is far below the HBM peak
-    projector = {k: {"avg_launch_ms": kern[k]["avg_ms"], "bytes_per_launch": kern[k]["bytes"],
-                     "achieved": kern[k]["gbs"], "unit": "GB/s",
-                     "frac": kern[k]["gbs"] / HBM_PEAK_GBS, "traffic": counted.get(k)}
-                 for k in ("fwd_single_kernel", "adj_single_kernel") if k in kern}
+    # and LDS work, so their byte rate is far below the HBM peak; their vector
+    # rate counts the useful multiply-adds, two per pixel, angle and slice
+    # (each pixel meets two bins per angle), against the non-packed FP32 peak
+    fma = 2.0 * prob.nx * prob.ny * prob.na * prob.C * prob.Nz
+    projector = {}
+    for k in ("fwd_single_kernel", "adj_single_kernel"):
+        if k not in kern:
+            continue
+        tflops = 2.0 * fma / (kern[k]["avg_ms"] * 1e-3) / 1e12
+        projector[k] = {"avg_launch_ms": kern[k]["avg_ms"], "bytes_per_launch": kern[k]["bytes"],
+                        "achieved": kern[k]["gbs"], "unit": "GB/s",
+                        "frac": kern[k]["gbs"] / HBM_PEAK_GBS, "traffic": counted.get(k),
+                        "valu": {"useful_fma_per_launch": fma, "achieved": tflops,
+                                 "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": tflops / VALU_PEAK_TFLOPS}}
 
     line = {
         "metric": "voxel-iters/s (coupled TGV-KL) + time-to-1e-3 rel err: warm-start vs plain PD",

@@ -478,8 +478,31 @@ int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
     if (lanes > 1) {
         int rc = plan_lanes(pl);
         if (rc) return rc;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GR_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs == hipStreamCaptureStatusInvalidated) {
+            set_error("plan: the capturing stream's capture was invalidated");
+            return GR_ERR_HIP;
+        }
         GR_HIP(hipEventRecord(pl->fork_ev, st));
-        for (int j = 1; j < lanes; ++j) GR_HIP(hipStreamWaitEvent(pl->side[j - 1], pl->fork_ev, 0));
+        for (int j = 1; j < lanes; ++j) {
+            GR_HIP(hipStreamWaitEvent(pl->side[j - 1], pl->fork_ev, 0));
+            if (cs == hipStreamCaptureStatusActive) {
+                // a side stream must have joined the capture through the fork
+                // event (recorded inside it); anything else would leave work
+                // or a dependency outside the graph
+                hipStreamCaptureStatus ss = hipStreamCaptureStatusNone;
+                GR_HIP(hipStreamIsCapturing(pl->side[j - 1], &ss));
+                if (ss != hipStreamCaptureStatusActive) {
+                    for (int i = 1; i < j; ++i) {  // join the lanes already forked
+                        (void)hipEventRecord(pl->join_ev[i - 1], pl->side[i - 1]);
+                        (void)hipStreamWaitEvent(st, pl->join_ev[i - 1], 0);
+                    }
+                    set_error("plan: side stream %d did not join the capture", j);
+                    return GR_ERR_UNSUPPORTED;
+                }
+            }
+        }
     }
     int rc = GR_OK;
     for (size_t i = 0; i < nl && !rc; ++i) {
@@ -929,6 +952,18 @@ int gr_plan_iterate_graph(gr_plan* pl, int n_iter, int block, void* stream)
         return GR_ERR_INVALID;
     }
     hipStream_t st = stream_of(stream);
+    {
+        // the graph is captured on a private stream and replayed on `stream`;
+        // replaying into a stream that is itself being captured would nest a
+        // captured graph launch, which this entry point does not do
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GR_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            set_error("plan_iterate_graph: the caller's stream is being captured; use "
+                      "gr_plan_iterate inside a capture");
+            return GR_ERR_UNSUPPORTED;
+        }
+    }
     if (pl->graph && pl->graph_block != block) {
         (void)hipGraphExecDestroy(pl->graph);
         pl->graph = nullptr;
