@@ -220,17 +220,36 @@ def batched_main(args):
     print(json.dumps(line), flush=True)
 
 
+def warm_start_summary(r):
+    """The warm-start object of a bench line from warmstart.evaluate()."""
+    t = r["targets"]
+    best = next((k for k in ("0.001", "0.003", "0.01") if t[k]["time_ratio_all"]), None)
+    return {
+        "value": t["0.001"]["time_ratio_all"],
+        "unit": "x (plain PD time / (warm-start inference + PD) time until every slice is "
+                "within 1e-3 relative L2 of u*)",
+        "tightest_tolerance_both_reach": best,
+        "value_at_tightest": t[best]["time_ratio_all"] if best else None,
+        "slices": r["slices"], "seeds": r["seeds"],
+        "targets": t, "error_median": r["error_median"],
+        "ustar": {"iterations": r["ustar_iters"],
+                  "largest_rel_change_last_1000": r["ustar_last1000_rel_change_max"]},
+        "inference_ms": r["inference_ms"], "iteration_ms_batch": r["iteration_ms_batch"],
+        "check_every": r["check_every"], "max_iters": r["max_iters"],
+    }
+
+
 def warmstart_main(args):
     """Config: learned warm start.  A 4-level U-Net (base 32, ~2M parameters,
-    BF16 MFMA kernels, FP32 accumulation) is trained on the solver's
+    BF16 MFMA kernels, FP32 accumulation) trained on the solver's
     reconstructions of seeded 4-channel 256 x 256 ellipse slices (60 tilts in
     [-75, 75] deg, 256 bins, Poisson counts scaled as the reference's caller
-    does, Frobenius-coupled TGV-KL) and evaluated on unseen slices against
-    u* (a long solver run): iterations and HIP-event time (error checks
-    excluded; backprojection + inference included for the warm start) to
-    reach plain primal-dual's own error after --budget-iters iterations and
-    fixed tolerances."""
-    import numpy as np
+    does, Frobenius-coupled TGV-KL).  --ws-stage train: make the training
+    pairs with the GPU solver, train, save the weights (--save-net).
+    --ws-stage eval: load the weights (--weights, default the committed ones)
+    and compare warm start + PD with plain PD on unseen slices against u*
+    (the plain iteration run until its change over 1000 iterations is below
+    --ustar-tol): iterations and HIP-event time to 1e-2, 3e-3 and 1e-3."""
     import torch
 
     from graptor_amd import warmstart as ws
@@ -238,34 +257,33 @@ def warmstart_main(args):
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     log = lambda msg: print(f"# {msg}", file=sys.stderr, flush=True)
-    r = ws.study(n_train=args.train_pairs, n_eval=args.eval_slices,
-                 target_iters=args.target_iters, ustar_iters=args.ustar_iters,
-                 train_steps=args.train_steps, budget_iters=args.budget_iters,
-                 max_iters=args.max_iters, device=dev, log=log)
-    net = r.pop("net")
-    if args.save_net:
-        np.savez(args.save_net, params=net.p.cpu().numpy())
-    r["losses"] = [(int(s), float(l)) for s, l in r["losses"]][-5:]
-    # headline: the tightest tolerance both methods reach for every slice
-    # within the iteration cap (1e-3 when reached)
-    reached = [k for k in ("0.001", "0.003", "0.01")
-               if r["targets"][k]["time_ratio_all"] is not None]
-    tol = reached[0] if reached else None
+    if args.ws_stage == "train":
+        net, info = ws.train_stage(n_train=args.train_pairs, target_iters=args.target_iters,
+                                   train_steps=args.train_steps, device=dev, log=log)
+        path = args.save_net or ws.WEIGHTS_PATH
+        ws.save_weights(net, path, meta=info)
+        print(json.dumps({"stage": "train", "weights": path, **info}), flush=True)
+        return
+    net = ws.load_weights(args.weights or ws.WEIGHTS_PATH, device=dev)
+    seeds = range(args.ws_seed0, args.ws_seed0 + args.eval_slices)
+    r = ws.evaluate(net, seeds=seeds, check_every=args.check_every, max_iters=args.max_iters,
+                    ustar_min=args.ustar_iters, ustar_tol=args.ustar_tol,
+                    ustar_cap=args.ustar_cap, device=dev, log=log)
+    summ = warm_start_summary(r)
     line = {
         "metric": "time-to-1e-3 rel err: warm-start vs plain PD",
-        "value": r["targets"][tol]["time_ratio_all"] if tol else None,
-        "unit": (f"x (plain time / warm-start time until every slice is within {tol} of u*; "
-                 "1e-3 not reached by either method within the cap)" if tol and tol != "0.001"
-                 else "x (plain time / warm-start time until every slice is within 1e-3 of u*)"),
+        "value": summ["value"], "unit": summ["unit"],
         "n_gpus": 1, "steps": args.max_iters, "warmup": 0,
         "ms_per_step": r["iteration_ms_batch"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 solver, bf16 network (fp32 accumulate)",
         "data": "synthetic (seeded 4-channel ellipse slices, Poisson counts, percentile-scaled)",
-        "config": {"workload": f"warm start: U-Net trained on {args.train_pairs} solver "
-                               f"reconstructions ({args.target_iters} it), evaluated on "
-                               f"{args.eval_slices} unseen 256x256x4 slices, 60 tilts; u* = "
-                               f"{args.ustar_iters} it; budget {args.budget_iters} it"},
-        "study": r,
+        "config": {"workload": f"warm start on {args.eval_slices} unseen 256x256x4 slices "
+                               f"(seeds {seeds[0]}..{seeds[-1]}), 60 tilts, Frobenius TGV-KL; "
+                               f"u* until change <= {args.ustar_tol:g} per 1000 it "
+                               f"(>= {args.ustar_iters} it)",
+                   "weights": os.path.relpath(args.weights or ws.WEIGHTS_PATH, ROOT),
+                   "weights_meta": getattr(net, "meta", {})},
+        "warm_start": summ,
     }
     print(json.dumps(line, default=float), flush=True)
 
@@ -519,6 +537,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000, help="timed iterations")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-iters", type=int, default=50)
+    ap.add_argument("--ramp-iters", type=int, default=200,
+                    help="untimed set-up iterations ahead of the profiled pass and warm-up")
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nz", type=int, default=64)
@@ -532,10 +552,21 @@ def main():
     ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--eval-slices", type=int, default=256)
     ap.add_argument("--target-iters", type=int, default=2500)
-    ap.add_argument("--ustar-iters", type=int, default=20000)
+    ap.add_argument("--ustar-iters", type=int, default=100000,
+                    help="least iterations of the u* run")
     ap.add_argument("--budget-iters", type=int, default=2500)
-    ap.add_argument("--max-iters", type=int, default=5000)
+    ap.add_argument("--max-iters", type=int, default=60000,
+                    help="iteration cap of the plain and warm-started arms")
     ap.add_argument("--save-net", default=None, help="write the trained weights (.npz)")
+    ap.add_argument("--weights", default=None, help="warm-start weights (.npz)")
+    ap.add_argument("--ws-stage", default="eval", choices=["train", "eval"])
+    ap.add_argument("--ws-seed0", type=int, default=60000)
+    ap.add_argument("--ws-slices", type=int, default=16,
+                    help="slices of the warm-start comparison in the default line")
+    ap.add_argument("--ustar-tol", type=float, default=1e-5)
+    ap.add_argument("--ustar-cap", type=int, default=200000)
+    ap.add_argument("--check-every", type=int, default=250)
+    ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--nprob", type=int, default=64)
     ap.add_argument("--check", type=int, default=8)
     ap.add_argument("--check-iters", type=int, default=50)
@@ -577,9 +608,11 @@ def main():
     del f
     print(f"# problem ready: {prob.C}x{prob.Nz}x{prob.ny}x{prob.nx}, na={prob.na}, "
           f"L={prob.k.L:.6f}", file=sys.stderr, flush=True)
-    # per-kernel durations: HIP events around every launch of a profiled pass
-    # (one-stream order); run ahead of the warm-up so that the timed steps
-    # start at the clocks of a busy GPU
+    # set-up ahead of the W warm-up steps: a fixed clock ramp (a 1000-iteration
+    # solve runs at the clocks of a busy GPU, the first few of a cold one do
+    # not), then the per-kernel durations, HIP events around every launch of a
+    # profiled pass (one-stream order)
+    prob.iterate(args.ramp_iters)
     ms = prob.profile(args.profile_iters)
     prob.iterate(args.warmup)
     torch.cuda.synchronize()
@@ -625,6 +658,25 @@ def main():
         cpu = cpu_baseline(f_host, geom.angles, prob.k.L, args.cpu_iters)
         print(f"# cpu baseline: {cpu}", file=sys.stderr, flush=True)
         cpu.pop("seconds_per_iter", None)
+
+    warm = None
+    if not args.no_warm_start and not large:
+        # the second half of the metric on a bounded sample: committed weights,
+        # --ws-slices unseen slices
+        from graptor_amd import warmstart as ws
+        wpath = args.weights or ws.WEIGHTS_PATH
+        if os.path.exists(wpath):
+            net = ws.load_weights(wpath, device=dev)
+            r = ws.evaluate(net, seeds=range(args.ws_seed0, args.ws_seed0 + args.ws_slices),
+                            check_every=args.check_every, max_iters=args.max_iters,
+                            ustar_min=args.ustar_iters, ustar_tol=args.ustar_tol,
+                            ustar_cap=args.ustar_cap, device=dev,
+                            log=lambda m: print(f"# {m}", file=sys.stderr, flush=True))
+            warm = warm_start_summary(r)
+            warm["weights"] = os.path.relpath(wpath, ROOT)
+            del net
+        else:
+            print(f"# no warm-start weights at {wpath}", file=sys.stderr, flush=True)
 
     converge = None
     if large and args.cap_iters > 0:
@@ -708,6 +760,8 @@ def main():
         "projector": dict(bound="valu+lds", **projector),
         "cpu_baseline": cpu,
     }
+    if warm is not None:
+        line["warm_start"] = warm
     if converge is not None:
         line["convergence"] = converge
     print(json.dumps(line), flush=True)

@@ -15,7 +15,9 @@ of shared buffers, so no copies are made.  torch only allocates memory.
 from __future__ import annotations
 
 import ctypes
+import json
 import math
+import os
 import time
 from dataclasses import dataclass
 
@@ -29,6 +31,10 @@ from .projector import RadonGeometry, make_function_feasible, radon_ad, radon_no
 from .solver import TgvProblem
 
 CIN_PAD = 16  # input channels padded to a multiple of 16 for the MFMA K step
+# weights trained by this package on its own solver's reconstructions of
+# 256 x 256 x 4 slices (bf16 bits, as inference uses them)
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                            "warmstart_unet_256x4.npz")
 
 
 def _lib_cnn():
@@ -671,4 +677,199 @@ def study(cfg: SliceConfig | None = None, n_train=4096, n_eval=256, target_iters
     out["iteration_ms_batch"] = per_iter_ms
     out["inference_tflops"] = conv_flops(net, N, N) * n_eval / (infer_ms * 1e-3) / 1e12
     out["net"] = net
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Stages of the warm-start workflow: training (solver-generated pairs), saved
+# weights, and the evaluation against a converged u*
+# ---------------------------------------------------------------------------
+def save_weights(net: UNet, path: str, meta: dict | None = None) -> None:
+    """The bf16 weights inference runs with (as uint16 bits) and the shape."""
+    bits = net.pb.detach().view(torch.int16).cpu().numpy().view(np.uint16)
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez_compressed(path, bf16=bits, cin=net.cin, cout=net.cout, base=net.base,
+                        meta=np.array(json.dumps(meta or {})))
+
+
+def load_weights(path: str = WEIGHTS_PATH, device="cuda") -> UNet:
+    """A UNet holding saved bf16 weights (FP32 master copy = the same values).
+    Loaded with numpy's default allow_pickle=False."""
+    with np.load(path) as d:
+        net = UNet(cin=int(d["cin"]), cout=int(d["cout"]), base=int(d["base"]), device=device)
+        bits = d["bf16"]
+        meta = json.loads(str(d["meta"]))
+    if bits.size != net.pb.numel():
+        raise ValueError(f"{path}: {bits.size} weights for a network of {net.pb.numel()}")
+    w = torch.as_tensor(bits.view(np.int16), device=net.dev).view(torch.bfloat16)
+    net.pb.copy_(w)
+    net.p.copy_(w.float())
+    net.meta = meta
+    return net
+
+
+def train_stage(cfg: SliceConfig | None = None, n_train=20000, target_iters=2500,
+                train_steps=12000, batch=16, lr=1e-3, train_seed0=40000, chunk=256,
+                device="cuda", log=None):
+    """Training pairs from the solver (slices seeded train_seed0 + i, targets
+    = u after ``target_iters`` iterations, divided by the data's channel
+    scales) and Adam training.  Returns (net, info)."""
+    cfg = cfg or SliceConfig()
+    log = log or (lambda *a: None)
+    dev = torch.device(device)
+    N, C = cfg.N, cfg.C
+    geom = cfg.geometry()
+    L = float(radon_normest(None, geom, mode="power", device=dev))
+    info = dict(L=L, n_train=n_train, target_iters=target_iters, train_seed0=train_seed0)
+    t0 = time.perf_counter()
+    X = torch.empty((n_train, N, N, C), device=dev)
+    T = torch.empty((n_train, N, N, C), device=dev)
+    for i in range(0, n_train, chunk):
+        m = min(chunk, n_train - i)
+        f = make_data(cfg, geom, range(train_seed0 + i, train_seed0 + i + m),
+                      noise_seed=train_seed0 + 1_000_000 + i, device=dev)
+        x, scale = features(f, geom)
+        u = solve(cfg, geom, f, L, target_iters, chunk=chunk)
+        X[i:i + m] = x
+        T[i:i + m] = (u / scale.view(m, C, 1, 1)).permute(0, 2, 3, 1)
+        log(f"training data {i + m}/{n_train} ({time.perf_counter() - t0:.1f} s)")
+        del f, x, u
+    torch.cuda.synchronize()
+    info["data_s"] = time.perf_counter() - t0
+    net = UNet(cin=C, cout=C, base=32, device=dev, seed=0)
+    t0 = time.perf_counter()
+    losses = train(net, X, T, train_steps, batch=batch, lr=lr,
+                   log=lambda s_, l_: log(f"step {s_}/{train_steps} loss {l_:.3e}"))
+    torch.cuda.synchronize()
+    info["train_s"] = time.perf_counter() - t0
+    info["train_steps"] = train_steps
+    info["train_tflops"] = 3.0 * conv_flops(net, N, N) * train_steps * batch / info["train_s"] / 1e12
+    info["final_loss"] = float(losses[-1][1]) if losses else None
+    del X, T
+    return net, info
+
+
+def converge(sb: SliceBatch, min_iters, tol, cap, block=5000, log=None, snap_every=0,
+             snap_until=0):
+    """Iterate ``sb`` until the largest per-slice relative change of u over
+    1000 iterations is <= tol (checked at the end of every ``block``
+    iterations, after at least ``min_iters`` and at most about ``cap``
+    iterations).  With ``snap_every`` the first ``snap_until`` iterations run
+    in blocks of ``snap_every`` and every iterate between blocks is kept on
+    the device, with the HIP-event time of the blocks up to it.  Returns (u,
+    iterations, last change, (its, ms, snapshots))."""
+    log = log or (lambda *a: None)
+    its, ms, snaps = [0], [0.0], ([sb.u().clone()] if snap_every else [])
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    it, t, change = 0, 0.0, float("inf")
+    while snap_every and it + snap_every <= snap_until:
+        e0.record()
+        sb.iterate(snap_every)
+        e1.record()
+        e1.synchronize()
+        t += e0.elapsed_time(e1)
+        it += snap_every
+        its.append(it)
+        ms.append(t)
+        snaps.append(sb.u().clone())
+        if it % 5000 == 0:
+            log(f"plain run: {it} iterations")
+    while it < cap:
+        sb.iterate(max(block - 1000, 10))
+        prev = sb.u().clone()
+        sb.iterate(1000)
+        it += max(block - 1000, 10) + 1000
+        change = float(rel_errors(prev, sb.u()).max())
+        log(f"u* run: {it} iterations, largest change over the last 1000 {change:.2e}")
+        if it >= min_iters and change <= tol:
+            break
+    return sb.u().clone(), it, change, (np.asarray(its), np.asarray(ms), snaps)
+
+
+def evaluate(net: UNet, cfg: SliceConfig | None = None, seeds=range(60000, 60016), L=None,
+             tols=(1e-2, 3e-3, 1e-3), check_every=250, max_iters=60000, ustar_min=100000,
+             ustar_tol=1e-5, ustar_cap=200000, device="cuda", log=None):
+    """Warm start + primal-dual against plain primal-dual on the slices
+    ``seeds``.  u* = the plain iteration run until its largest per-slice
+    change over 1000 iterations is <= ustar_tol (at least ustar_min, at most
+    ustar_cap iterations); the plain arm is that same run's iterates up to
+    max_iters, the warm arm a second run from the network's u0 (checked every
+    check_every iterations, stopped once every slice is within the smallest
+    tolerance of u*).  Times are HIP-event times of the iteration blocks (error
+    checks excluded), plus backprojection and inference for the warm arm.
+    Returns a dict of per-tolerance iterations and times."""
+    cfg = cfg or SliceConfig()
+    log = log or (lambda *a: None)
+    dev = torch.device(device)
+    geom = cfg.geometry()
+    if L is None:
+        L = float(radon_normest(None, geom, mode="power", device=dev))
+    seeds = list(seeds)
+    B = len(seeds)
+    f = make_data(cfg, geom, seeds, noise_seed=seeds[0] + 1_000_000, device=dev)
+    out = dict(L=L, slices=B, seeds=[seeds[0], seeds[-1]], check_every=check_every,
+               max_iters=max_iters)
+    # u* and the plain arm
+    t0 = time.perf_counter()
+    sb = SliceBatch(cfg, geom, f, L)
+    ustar, n_star, change, (its_p, ms_p, snaps) = converge(
+        sb, ustar_min, ustar_tol, ustar_cap, log=log, snap_every=check_every,
+        snap_until=max_iters)
+    del sb
+    out.update(ustar_iters=n_star, ustar_last1000_rel_change_max=change,
+               ustar_s=time.perf_counter() - t0)
+    err_p = np.stack([rel_errors(u, ustar).cpu().numpy() for u in snaps])
+    del snaps
+    per_iter_ms = float(ms_p[-1] / its_p[-1])
+    # warm arm
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    x, scale = features(f, geom)
+    u0 = predict(net, x, scale, batch=16 if B % 16 == 0 else 1)
+    e1.record()
+    e1.synchronize()
+    infer_ms = e0.elapsed_time(e1)
+    sb = SliceBatch(cfg, geom, f, L, u0=u0)
+    its_w, ms_w, errs_w = [0], [0.0], [rel_errors(sb.u(), ustar).cpu().numpy()]
+    it, t = 0, 0.0
+    while it < max_iters and errs_w[-1].max() > min(tols):
+        e0.record()
+        sb.iterate(check_every)
+        e1.record()
+        e1.synchronize()
+        t += e0.elapsed_time(e1)
+        it += check_every
+        its_w.append(it)
+        ms_w.append(t)
+        errs_w.append(rel_errors(sb.u(), ustar).cpu().numpy())
+        if it % 5000 == 0:
+            log(f"warm arm: {it} iterations, largest error {errs_w[-1].max():.2e}")
+    del sb
+    its_w, ms_w, err_w = np.asarray(its_w), np.asarray(ms_w) + infer_ms, np.stack(errs_w)
+
+    def med(a):
+        return float(np.median(a)) if len(a) else None
+
+    rows = {}
+    for tol in tols:
+        a, b = first_reach(its_p, err_p, tol), first_reach(its_w, err_w, tol)
+        both = (a >= 0) & (b >= 0)
+        r = dict(plain_reached=int((a >= 0).sum()), warm_reached=int((b >= 0).sum()),
+                 plain_iters_median=med(a[a >= 0]), warm_iters_median=med(b[b >= 0]),
+                 iter_ratio_median=med(a[both] / np.maximum(b[both], check_every)))
+        r["plain_ms_all"] = float(ms_p[its_p == a.max()][0]) if (a >= 0).all() else None
+        r["warm_ms_all"] = float(ms_w[its_w == b.max()][0]) if (b >= 0).all() else None
+        r["time_ratio_all"] = (r["plain_ms_all"] / r["warm_ms_all"]
+                               if r["plain_ms_all"] and r["warm_ms_all"] else None)
+        rows[f"{tol:g}"] = r
+    out["targets"] = rows
+    marks = [0, 100, 500, 1000, 2500, 5000, 10000, 20000, 40000, max_iters]
+    out["error_median"] = {
+        int(k): (med(err_p[np.searchsorted(its_p, k)]) if k <= its_p[-1] else None,
+                 med(err_w[np.searchsorted(its_w, k)]) if k <= its_w[-1] else None)
+        for k in marks}
+    out["inference_ms"] = infer_ms
+    out["iteration_ms_batch"] = per_iter_ms
+    out["per_slice"] = {"plain_iters_to_min_tol": first_reach(its_p, err_p, min(tols)).tolist(),
+                        "warm_iters_to_min_tol": first_reach(its_w, err_w, min(tols)).tolist()}
     return out
