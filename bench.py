@@ -233,7 +233,8 @@ def warm_start_summary(r):
         "slices": r["slices"], "seeds": r["seeds"],
         "targets": t, "error_median": r["error_median"],
         "ustar": {"iterations": r["ustar_iters"],
-                  "largest_rel_change_last_1000": r["ustar_last1000_rel_change_max"]},
+                  "largest_rel_change_last_1000": r["ustar_last1000_rel_change_max"],
+                  "largest_rel_distance_from_iteration": r.get("ustar_distance_to_earlier")},
         "inference_ms": r["inference_ms"], "iteration_ms_batch": r["iteration_ms_batch"],
         "check_every": r["check_every"], "max_iters": r["max_iters"],
     }

@@ -750,15 +750,22 @@ def train_stage(cfg: SliceConfig | None = None, n_train=20000, target_iters=2500
 
 
 def converge(sb: SliceBatch, min_iters, tol, cap, block=5000, log=None, snap_every=0,
-             snap_until=0):
+             snap_until=0, anchor_every=25000, drift=None):
     """Iterate ``sb`` until the largest per-slice relative change of u over
     1000 iterations is <= tol (checked at the end of every ``block``
     iterations, after at least ``min_iters`` and at most about ``cap``
     iterations).  With ``snap_every`` the first ``snap_until`` iterations run
     in blocks of ``snap_every`` and every iterate between blocks is kept on
-    the device, with the HIP-event time of the blocks up to it.  Returns (u,
-    iterations, last change, (its, ms, snapshots))."""
+    the device, with the HIP-event time of the blocks up to it.  Iterates at
+    multiples of ``anchor_every`` from min_iters / 2 on are kept as well; when
+    ``drift`` is a dict it receives, per anchor iteration, the largest
+    per-slice relative distance of the final u from it (a distance that stays
+    at the size of the last change, instead of growing with the number of
+    iterations between them, shows an iterate that fluctuates about a fixed
+    point rather than drifting).  Returns (u, iterations, last change, (its,
+    ms, snapshots))."""
     log = log or (lambda *a: None)
+    anchors = {}
     its, ms, snaps = [0], [0.0], ([sb.u().clone()] if snap_every else [])
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     it, t, change = 0, 0.0, float("inf")
@@ -783,7 +790,12 @@ def converge(sb: SliceBatch, min_iters, tol, cap, block=5000, log=None, snap_eve
         log(f"u* run: {it} iterations, largest change over the last 1000 {change:.2e}")
         if it >= min_iters and change <= tol:
             break
-    return sb.u().clone(), it, change, (np.asarray(its), np.asarray(ms), snaps)
+        if anchor_every and it % anchor_every == 0 and 2 * it >= min_iters:
+            anchors[it] = sb.u().clone()
+    u = sb.u().clone()
+    if drift is not None:
+        drift.update({k: float(rel_errors(a, u).max()) for k, a in anchors.items() if k < it})
+    return u, it, change, (np.asarray(its), np.asarray(ms), snaps)
 
 
 def evaluate(net: UNet, cfg: SliceConfig | None = None, seeds=range(60000, 60016), L=None,
@@ -812,12 +824,13 @@ def evaluate(net: UNet, cfg: SliceConfig | None = None, seeds=range(60000, 60016
     # u* and the plain arm
     t0 = time.perf_counter()
     sb = SliceBatch(cfg, geom, f, L)
+    drift = {}
     ustar, n_star, change, (its_p, ms_p, snaps) = converge(
         sb, ustar_min, ustar_tol, ustar_cap, log=log, snap_every=check_every,
-        snap_until=max_iters)
+        snap_until=max_iters, drift=drift)
     del sb
     out.update(ustar_iters=n_star, ustar_last1000_rel_change_max=change,
-               ustar_s=time.perf_counter() - t0)
+               ustar_distance_to_earlier=drift, ustar_s=time.perf_counter() - t0)
     err_p = np.stack([rel_errors(u, ustar).cpu().numpy() for u in snaps])
     del snaps
     per_iter_ms = float(ms_p[-1] / its_p[-1])
