@@ -114,10 +114,15 @@ def scale_data(data, section, scalepars, dim):
 
 
 def apply_baseline(sino, mode, threshold, averagenoise, dim):
-    """Baseline rules (2D RC:193-202: 'mean', 'meanthres', 'thresholding';
-    3D RC:364-373: 'mean1', 'mean2', 'thresholding').  The
-    'mean_threshold_bright' rule re-reads the file through the 'basic'
-    path; it is not supported here."""
+    """Baseline rules (2D RC:193-208: 'mean', 'meanthres', 'thresholding',
+    'mean_threshold_bright'; 3D RC:364-378: 'mean1', 'mean2', 'thresholding',
+    'mean_threshold_bright').  'mean_threshold_bright' zeroes values below
+    the threshold and then scales every projection (first axis) so that its
+    mean equals the mean of the projection means (Brightness_correction2,
+    H5:256-269).  The reference also re-reads the last channel's file there
+    (H5:273-312) and keeps only its angle list, which is the loaded one
+    whenever the brightness mode removed the same bad projections; the
+    loaded angles are kept here."""
     m = (mode or "").upper()
     sino = np.array(sino, dtype=np.float64, copy=True)
     if (dim == 2 and m == "MEAN") or (dim == 3 and m == "MEAN1"):
@@ -128,7 +133,10 @@ def apply_baseline(sino, mode, threshold, averagenoise, dim):
         sino[sino < threshold] = 0.0
         return sino
     if m == "MEAN_THRESHOLD_BRIGHT":
-        raise NotImplementedError("baseline 'mean_threshold_bright' is not supported")
+        sino[sino < threshold] = 0.0
+        means = sino.reshape(sino.shape[0], -1).mean(axis=1)
+        scale = means.mean() / means
+        return sino * scale.reshape((-1,) + (1,) * (sino.ndim - 1))
     return sino
 
 

@@ -86,8 +86,27 @@ def test_baselines():
     assert np.allclose(pl.apply_baseline(s, "mean2", 0.6, 0.0, 3), s - m)
     assert np.allclose(pl.apply_baseline(s, "meanthres", 0.6, 0.0, 2), s - m)
     assert np.allclose(pl.apply_baseline(s, "", 0.6, 0.0, 2), s)
-    with pytest.raises(NotImplementedError):
-        pl.apply_baseline(s, "mean_threshold_bright", 0.6, 0.0, 3)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mean_threshold_bright_equalises_projection_means(dim):
+    """Threshold, then every projection scaled to the mean of the projection
+    means (H5:256-269): checked by its defining properties, not its formula —
+    values below the threshold become 0, each projection is a positive
+    multiple of its thresholded self, and all projection means are equal to
+    the mean of the thresholded projection means."""
+    rng = np.random.default_rng(11)
+    shape = (7, 5) if dim == 2 else (7, 3, 5)
+    s = rng.uniform(0.0, 2.0, shape) * rng.uniform(0.2, 3.0, (7,) + (1,) * (dim - 1))
+    out = pl.apply_baseline(s, "mean_threshold_bright", 0.4, 0.0, dim)
+    t = np.where(s < 0.4, 0.0, s)
+    assert np.array_equal(out == 0, t == 0)
+    ratio = out.reshape(7, -1)[t.reshape(7, -1) > 0] / t.reshape(7, -1)[t.reshape(7, -1) > 0]
+    per = [np.unique(np.round((out[i] / np.where(t[i] > 0, t[i], 1))[t[i] > 0], 12))
+           for i in range(7)]
+    assert all(len(p) == 1 and p[0] > 0 for p in per) and ratio.min() > 0
+    target = np.mean([t[i].mean() for i in range(7)])
+    assert np.allclose([out[i].mean() for i in range(7)], target, rtol=1e-12)
 
 
 def test_faktor_rescale_restores_mass_ratio():
