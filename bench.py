@@ -265,8 +265,14 @@ def warmstart_main(args):
         ws.save_weights(net, path, meta=info)
         print(json.dumps({"stage": "train", "weights": path, **info}), flush=True)
         return
-    net = ws.load_weights(args.weights or ws.WEIGHTS_PATH, device=dev)
     seeds = range(args.ws_seed0, args.ws_seed0 + args.eval_slices)
+    if args.ws_stage == "ustar":
+        r = ws.ustar_check(seeds=seeds, ustar_iters=args.ustar_cap,
+                           ratios=[float(x) for x in args.step_ratios.split(",")],
+                           iters=args.ratio_iters, device=dev, log=log)
+        print(json.dumps({"stage": "ustar", **r}, default=float), flush=True)
+        return
+    net = ws.load_weights(args.weights or ws.WEIGHTS_PATH, device=dev)
     r = ws.evaluate(net, seeds=seeds, check_every=args.check_every, max_iters=args.max_iters,
                     ustar_min=args.ustar_iters, ustar_tol=args.ustar_tol,
                     ustar_cap=args.ustar_cap, device=dev, log=log)
@@ -560,7 +566,11 @@ def main():
                     help="iteration cap of the plain and warm-started arms")
     ap.add_argument("--save-net", default=None, help="write the trained weights (.npz)")
     ap.add_argument("--weights", default=None, help="warm-start weights (.npz)")
-    ap.add_argument("--ws-stage", default="eval", choices=["train", "eval"])
+    ap.add_argument("--ws-stage", default="eval", choices=["train", "eval", "ustar"],
+                    help="ustar: distance of a --ustar-cap-iteration u* to runs with other "
+                         "primal/dual step ratios (same fixed point)")
+    ap.add_argument("--step-ratios", default="0.25,0.5,2,4")
+    ap.add_argument("--ratio-iters", type=int, default=120000)
     ap.add_argument("--ws-seed0", type=int, default=60000)
     ap.add_argument("--ws-slices", type=int, default=16,
                     help="slices of the warm-start comparison in the default line")

@@ -422,9 +422,13 @@ def features(f, geom):
 
 
 class SliceBatch:
-    """B independent 2D problems of one geometry advanced by one batch plan."""
+    """B independent 2D problems of one geometry advanced by one batch plan.
+    ``step_ratio`` s != 1 replaces the reference's steps sigma = tau =
+    1/sqrt(Lsqr) by sigma = s/sqrt(Lsqr), tau = 1/(s sqrt(Lsqr)) (the same
+    product, so the same convergence condition and the same fixed point);
+    only the u* check uses it."""
 
-    def __init__(self, cfg: SliceConfig, geom, f, L, u0=None):
+    def __init__(self, cfg: SliceConfig, geom, f, L, u0=None, step_ratio=1.0):
         B = f.shape[0]
         self.cfg, self.B = cfg, B
         self.problems = [
@@ -434,7 +438,18 @@ class SliceBatch:
                        u0=None if u0 is None else u0[b].view(cfg.C, 1, cfg.N, cfg.N),
                        device=f.device, fused=False)
             for b in range(B)]
+        if step_ratio != 1.0:
+            if cfg.discrepancy != "KL":
+                raise ValueError("step_ratio: the L2 prox factor depends on sigma")
+            for p in self.problems:
+                s0 = p.params.sigma
+                p.params.sigma, p.params.tau = s0 * step_ratio, s0 / step_ratio
+                p.params.sigma_p = p.params.sigma / L
         self.plan = BatchPlan(self.problems)
+
+    def energy(self):
+        """FP64 primal energy of every slice at its current (u, p)."""
+        return [p.energy()[2] for p in self.problems]
 
     def iterate(self, n, block=10):
         if n % block == 0:
@@ -796,6 +811,51 @@ def converge(sb: SliceBatch, min_iters, tol, cap, block=5000, log=None, snap_eve
     if drift is not None:
         drift.update({k: float(rel_errors(a, u).max()) for k, a in anchors.items() if k < it})
     return u, it, change, (np.asarray(its), np.asarray(ms), snaps)
+
+
+def ustar_check(cfg: SliceConfig | None = None, seeds=range(60000, 60016), L=None,
+                ustar_iters=300000, ratios=(0.25, 0.5, 2.0, 4.0), iters=120000, every=10000,
+                device="cuda", log=None):
+    """How close a plain-iteration u* is to the fixed point: u* = ``ustar_iters``
+    plain iterations from zero; then, for each step ratio s, a run from zero
+    with sigma = s/sqrt(Lsqr), tau = 1/(s sqrt(Lsqr)) (same fixed point),
+    recording every ``every`` iterations its largest and median per-slice
+    relative distance to u* and its change since the previous record, and
+    at the end its FP64 energy minus u*'s, per slice."""
+    cfg = cfg or SliceConfig()
+    log = log or (lambda *a: None)
+    dev = torch.device(device)
+    geom = cfg.geometry()
+    if L is None:
+        L = float(radon_normest(None, geom, mode="power", device=dev))
+    seeds = list(seeds)
+    f = make_data(cfg, geom, seeds, noise_seed=seeds[0] + 1_000_000, device=dev)
+    sb = SliceBatch(cfg, geom, f, L)
+    for it in range(every, ustar_iters + 1, every):
+        sb.iterate(every)
+        if it % 50000 == 0:
+            log(f"u* run: {it} iterations")
+    ustar, e_star = sb.u().clone(), sb.energy()
+    del sb
+    out = dict(L=L, slices=len(seeds), seeds=[seeds[0], seeds[-1]], ustar_iters=ustar_iters,
+               ustar_energy=e_star, runs={})
+    for s in ratios:
+        sb = SliceBatch(cfg, geom, f, L, step_ratio=s)
+        rows, prev = [], None
+        for it in range(every, iters + 1, every):
+            sb.iterate(every)
+            u = sb.u()
+            d = rel_errors(u, ustar)
+            rows.append(dict(iterations=it, max_dist=float(d.max()),
+                             median_dist=float(d.median()),
+                             max_change=None if prev is None
+                             else float(rel_errors(prev, u).max())))
+            prev = u.clone()
+            log(f"step ratio {s:g}: {it} iterations, largest distance to u* {rows[-1]['max_dist']:.2e}")
+        de = [a - b for a, b in zip(sb.energy(), e_star)]
+        out["runs"][f"{s:g}"] = dict(rows=rows, energy_minus_ustar=de)
+        del sb
+    return out
 
 
 def evaluate(net: UNet, cfg: SliceConfig | None = None, seeds=range(60000, 60016), L=None,

@@ -53,3 +53,34 @@ def test_flip_augmentation_is_a_symmetry_of_the_data():
         x1f = x1.flip([d - 1 for d in dims])
         err = (x1f - x2).norm() / x2.norm()
         assert err < 1e-5, (dims, float(err))
+
+
+def test_step_ratio_runs_share_the_fixed_point():
+    """SliceBatch(step_ratio=s) keeps sigma * tau (so the convergence
+    condition) and moves toward the same u as the reference steps: after many
+    iterations the runs with s = 1 and s = 2 are far closer to each other than
+    either is to its start or to its 200-iteration iterate."""
+    from graptor_amd import warmstart as ws
+    cfg = ws.SliceConfig(N=32, na=20)
+    geom = cfg.geometry()
+    f = ws.make_data(cfg, geom, [60000, 60001], noise_seed=7, device="cuda")
+    L = 40.0
+    one = ws.SliceBatch(cfg, geom, f, L)
+    two = ws.SliceBatch(cfg, geom, f, L, step_ratio=2.0)
+    for a, b in zip(one.problems, two.problems):
+        assert b.params.sigma == pytest.approx(2 * a.params.sigma, rel=1e-6)
+        assert b.params.tau == pytest.approx(a.params.tau / 2, rel=1e-6)
+        assert b.params.sigma * b.params.tau == pytest.approx(a.params.sigma * a.params.tau,
+                                                             rel=1e-6)
+        assert b.params.sigma_p == pytest.approx(b.params.sigma / L, rel=1e-6)
+    one.iterate(200)
+    two.iterate(200)
+    early = float(ws.rel_errors(one.u(), two.u()).max())
+    one.iterate(20000)
+    two.iterate(20000)
+    late = float(ws.rel_errors(one.u(), two.u()).max())
+    print(f"distance s=1 vs s=2: {early:.2e} after 200, {late:.2e} after 20200", flush=True)
+    assert torch.isfinite(one.u()).all() and torch.isfinite(two.u()).all()
+    assert early > 1e-3 and late < 0.2 * early
+    with pytest.raises(ValueError):
+        ws.SliceBatch(ws.SliceConfig(N=32, na=20, discrepancy="L2"), geom, f, L, step_ratio=2.0)
