@@ -268,7 +268,7 @@ def warmstart_main(args):
     seeds = range(args.ws_seed0, args.ws_seed0 + args.eval_slices)
     if args.ws_stage == "ustar":
         r = ws.ustar_check(seeds=seeds, ustar_iters=args.ustar_cap,
-                           ratios=[float(x) for x in args.step_ratios.split(",")],
+                           ratios=[float(x) for x in args.step_ratios.split(",") if x],
                            iters=args.ratio_iters, device=dev, log=log)
         print(json.dumps({"stage": "ustar", **r}, default=float), flush=True)
         return
