@@ -426,7 +426,8 @@ int plan_build(gr_plan* pl, const gr_geometry* const* geoms)
                         plp.items.push_back(DualItem{i, x0, y0, sl});
         }
         // dual items: 64 x 4 tiles of every plane
-        const int maxc = P.nch <= 4 ? 4 : P.nch <= 8 ? 8 : P.nch;  // > 8 rejected at launch
+        // 4 / 8: channels held in registers; 9: any larger count, swept twice
+        const int maxc = P.nch <= 4 ? 4 : P.nch <= 8 ? 8 : 9;
         auto dk = std::make_tuple(P.dim, P.coupling, P.tgv, maxc);
         if (!dual_idx.count(dk)) {
             dual_idx[dk] = pl->dual.size();
@@ -893,12 +894,6 @@ int gr_plan_create(int nprob, const gr_geometry* const* geoms, const gr_tgv_para
         }
         if (has_halos(&states[i])) {
             set_error("plan_create: problem %d has z halo planes (slabs run singly)", i);
-            return GR_ERR_UNSUPPORTED;
-        }
-        if (prms[i].nch > 8) {
-            // the batched dual kernel holds at most 8 channels of a voxel
-            set_error("plan_create: problem %d has %d channels; batches hold at most 8", i,
-                      prms[i].nch);
             return GR_ERR_UNSUPPORTED;
         }
     }

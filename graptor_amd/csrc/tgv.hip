@@ -94,8 +94,164 @@ struct FusedSrc {
     }
 };
 
+// One channel's dual ascent at one voxel, before any projection:
+// vv = v + sigma (grad ubar - pbar), ww = w + sigma E(pbar).  Returns the
+// weighted squared norm of ww (2D:232, 3D:582; 0 without TGV).  v and w are
+// read at the voxel itself only; ubar and pbar also at its neighbours (x
+// neighbours from the adjacent lanes of a 64-wide row).
+template <int DIM, int CPL, bool TGV, class Src, bool ZH>
+__device__ __forceinline__ float dual_candidate(const ProbDev& P, const Src& src, const float* v,
+                                                const float* w, int x, int y, int z, size_t i,
+                                                int c, float (&vv)[DIM],
+                                                float (&ww)[TGV ? (DIM == 2 ? 3 : 6) : 1])
+{
+    constexpr int D = DIM;
+    constexpr int M = DIM == 2 ? 3 : 6;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nch = P.nch;
+    const float hinv = P.hinv, sigma = P.sigma;
+    const size_t plane = (size_t)nx * ny;
+    const size_t chs = plane * nz;        // channel stride
+    const size_t cs = chs * nch;          // component stride
+    const size_t ic = i + c * chs;
+    float u0;
+    float p[3] = {0.f, 0.f, 0.f};
+    src.own(c, ic, x, y, z, u0, p);
+    float g[3];
+    if constexpr (Src::lanex) {
+        float un = lane_next(u0);
+        if ((threadIdx.x & 63) == 63 && x < nx - 1) un = src.ubn(c, ic + 1, x + 1, y, z);
+        g[0] = (x < nx - 1) ? un - u0 : 0.f;
+    } else {
+        g[0] = (x < nx - 1) ? src.ubn(c, ic + 1, x + 1, y, z) - u0 : 0.f;
+    }
+    g[1] = (y < ny - 1) ? src.ubn(c, ic + nx, x, y + 1, z) - u0 : 0.f;
+    if constexpr (DIM == 3) {
+        if constexpr (ZH) {
+            // stored ubar; above a slab's last plane the halo plane, if any
+            const float* up = z < nz - 1 ? src.ubar + ic + plane
+                              : P.halo_ubar_hi
+                                  ? P.halo_ubar_hi + (size_t)c * plane + (size_t)y * nx + x
+                                  : nullptr;
+            g[2] = up ? (*up - u0) * hinv : 0.f;
+        } else {
+            g[2] = (z < nz - 1) ? (src.ubn(c, ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) vv[k] = v[ic + k * cs] + sigma * (g[k] - p[k]);
+    float n2 = 0.f;
+    if constexpr (TGV) {
+        float ax[3] = {0.f, 0.f, 0.f}, ay[3] = {0.f, 0.f, 0.f}, az[3] = {0.f, 0.f, 0.f};
+        float q[3];
+        if constexpr (Src::lanex) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) q[k] = lane_prev(p[k]);
+            if ((threadIdx.x & 63) == 0 && x > 0) src.pbn(c, ic - 1, x - 1, y, z, q);
+            if (x > 0) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
+            }
+        } else if (x > 0) {
+            src.pbn(c, ic - 1, x - 1, y, z, q);
+#pragma unroll
+            for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
+        }
+        if (y > 0) {
+            src.pbn(c, ic - nx, x, y - 1, z, q);
+#pragma unroll
+            for (int k = 0; k < D; ++k) ay[k] = p[k] - q[k];
+        }
+        if constexpr (DIM == 3) {
+            if constexpr (ZH) {
+                // stored pbar; below a slab's first plane the halo plane, if any
+                const float* dn = z > 0 ? src.pbar + ic - plane
+                                  : P.halo_pbar_lo
+                                      ? P.halo_pbar_lo + (size_t)c * plane + (size_t)y * nx + x
+                                      : nullptr;
+                const size_t kst = z > 0 ? cs : (size_t)nch * plane;
+                if (dn) {
+#pragma unroll
+                    for (int k = 0; k < D; ++k) az[k] = (p[k] - dn[k * kst]) * hinv;
+                }
+            } else if (z > 0) {
+                src.pbn(c, ic - plane, x, y, z - 1, q);
+#pragma unroll
+                for (int k = 0; k < D; ++k) az[k] = (p[k] - q[k]) * hinv;
+            }
+        }
+        float e[6];
+        e[0] = ax[0];
+        e[1] = ay[1];
+        if constexpr (DIM == 2) {
+            e[2] = 0.5f * (ax[1] + ay[0]);
+        } else {
+            e[2] = az[2];
+            e[3] = 0.5f * (ax[1] + ay[0]);
+            e[4] = 0.5f * (ax[2] + az[0]);
+            e[5] = 0.5f * (ay[2] + az[1]);
+        }
+#pragma unroll
+        for (int m = 0; m < M; ++m) ww[m] = w[ic + m * cs] + sigma * e[m];
+        // weighted squared norm of this channel's tensor (2D:232, 3D:582)
+        if constexpr (DIM == 2) {
+            const float od = CPL == GR_COUPLING_UNCORRELATED ? W_OFF * ww[2] : 2.f * ww[2];
+            n2 = ww[0] * ww[0] + ww[1] * ww[1] + od * od;
+        } else {
+            const float o3 = W_OFF * ww[3], o4 = W_OFF * ww[4];
+            const float o5 = CPL == GR_COUPLING_UNCORRELATED ? ww[5] : W_OFF * ww[5];
+            n2 = ww[0] * ww[0] + ww[1] * ww[1] + ww[2] * ww[2] + o3 * o3 + o4 * o4 + o5 * o5;
+        }
+    }
+    return n2;
+}
+
+// Per-channel projections of the uncorrelated coupling (2D:98-128,
+// 3D:553-570, 597-613): each channel's v onto its own ball, each channel's
+// w (weighted norm n2w) onto its own.
+template <int DIM, bool TGV>
+__device__ __forceinline__ void project_channel(float (&vv)[DIM],
+                                                float (&ww)[TGV ? (DIM == 2 ? 3 : 6) : 1],
+                                                float n2w, float ainv0, float ainv1)
+{
+    constexpr int M = DIM == 2 ? 3 : 6;
+    if constexpr (TGV) {
+        const float n = sqrtf(n2w) * ainv0;
+        const float sc = n > 1.f ? 1.f / n : 1.f;
+#pragma unroll
+        for (int m = 0; m < M; ++m) ww[m] *= sc;
+    }
+    float n2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) n2 += vv[k] * vv[k];
+    const float n = sqrtf(n2) * ainv1;
+    const float sc = n > 1.f ? 1.f / n : 1.f;
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) vv[k] *= sc;
+}
+
+// The 3D nuclear-norm projection of a voxel's v (3D:290-527) as a 3x3 matrix
+// from the Gram matrix G = sum_c v_c v_c^T (xx, yy, zz, xy, xz, yz): every
+// singular value s of [v_c] maps to min(s, 1/ainv1), i.e. v_c <- Mx^T v_c.
+__device__ __forceinline__ void nuclear_matrix3(const float (&G)[6], float ainv1,
+                                                float (&Mx)[3][3])
+{
+    float ev[3], V[3][3], S[3];
+    jacobi3<float>(G, ev, V);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float sg = ev[k] > 0.f ? sqrtf(ev[k]) : 0.f;
+        S[k] = sg * ainv1 > 1.f ? 1.f / (ainv1 * sg) : 1.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            Mx[r][q] = S[0] * V[r][0] * V[q][0] + S[1] * V[r][1] * V[q][1] +
+                       S[2] * V[r][2] * V[q][2];
+}
+
 // Dual step at one voxel, all channels (update_v/update_w + the coupled
-// projections): v = proj(v_in + sigma (grad ubar - pbar)), w = proj(w_in +
+// projections): v = proj(v_in + sigma (grad ubar - pbar), w = proj(w_in +
 // sigma E pbar), written to v_out / w_out (the same arrays as v / w for the
 // standalone step: each thread reads its own entries before writing them,
 // so they are not declared restrict).
@@ -107,7 +263,7 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
     constexpr int D = DIM;
     constexpr int M = DIM == 2 ? 3 : 6;
     const int nx = P.nx, ny = P.ny, nz = P.nz, nch = P.nch;
-    const float hinv = P.hinv, sigma = P.sigma, ainv0 = P.ainv0, ainv1 = P.ainv1;
+    const float ainv0 = P.ainv0, ainv1 = P.ainv1;
     if (x >= nx || y >= ny) return;
     const size_t plane = (size_t)nx * ny;
     const size_t chs = plane * nz;        // channel stride
@@ -120,119 +276,14 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) {
         if (c >= nch) break;
-        const size_t ic = i + c * chs;
-        float u0;
-        float p[3] = {0.f, 0.f, 0.f};
-        src.own(c, ic, x, y, z, u0, p);
-        float g[3];
-        if constexpr (Src::lanex) {
-            float un = lane_next(u0);
-            if ((threadIdx.x & 63) == 63 && x < nx - 1) un = src.ubn(c, ic + 1, x + 1, y, z);
-            g[0] = (x < nx - 1) ? un - u0 : 0.f;
-        } else {
-            g[0] = (x < nx - 1) ? src.ubn(c, ic + 1, x + 1, y, z) - u0 : 0.f;
-        }
-        g[1] = (y < ny - 1) ? src.ubn(c, ic + nx, x, y + 1, z) - u0 : 0.f;
-        if constexpr (DIM == 3) {
-            if constexpr (ZH) {
-                // stored ubar; above a slab's last plane the halo plane, if any
-                const float* up = z < nz - 1 ? src.ubar + ic + plane
-                                  : P.halo_ubar_hi
-                                      ? P.halo_ubar_hi + (size_t)c * plane + (size_t)y * nx + x
-                                      : nullptr;
-                g[2] = up ? (*up - u0) * hinv : 0.f;
-            } else {
-                g[2] = (z < nz - 1) ? (src.ubn(c, ic + plane, x, y, z + 1) - u0) * hinv : 0.f;
-            }
-        }
+        const float n2 =
+            dual_candidate<DIM, CPL, TGV, Src, ZH>(P, src, v, w, x, y, z, i, c, vv[c], ww[c]);
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            vv[c][k] = v[ic + k * cs] + sigma * (g[k] - p[k]);
-            nv2 += vv[c][k] * vv[c][k];
-        }
-        if constexpr (TGV) {
-            float ax[3] = {0.f, 0.f, 0.f}, ay[3] = {0.f, 0.f, 0.f}, az[3] = {0.f, 0.f, 0.f};
-            float q[3];
-            if constexpr (Src::lanex) {
-#pragma unroll
-                for (int k = 0; k < D; ++k) q[k] = lane_prev(p[k]);
-                if ((threadIdx.x & 63) == 0 && x > 0) src.pbn(c, ic - 1, x - 1, y, z, q);
-                if (x > 0) {
-#pragma unroll
-                    for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
-                }
-            } else if (x > 0) {
-                src.pbn(c, ic - 1, x - 1, y, z, q);
-#pragma unroll
-                for (int k = 0; k < D; ++k) ax[k] = p[k] - q[k];
-            }
-            if (y > 0) {
-                src.pbn(c, ic - nx, x, y - 1, z, q);
-#pragma unroll
-                for (int k = 0; k < D; ++k) ay[k] = p[k] - q[k];
-            }
-            if constexpr (DIM == 3) {
-                if constexpr (ZH) {
-                    // stored pbar; below a slab's first plane the halo plane, if any
-                    const float* dn = z > 0 ? src.pbar + ic - plane
-                                      : P.halo_pbar_lo
-                                          ? P.halo_pbar_lo + (size_t)c * plane +
-                                                (size_t)y * nx + x
-                                          : nullptr;
-                    const size_t kst = z > 0 ? cs : (size_t)nch * plane;
-                    if (dn) {
-#pragma unroll
-                        for (int k = 0; k < D; ++k) az[k] = (p[k] - dn[k * kst]) * hinv;
-                    }
-                } else if (z > 0) {
-                    src.pbn(c, ic - plane, x, y, z - 1, q);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) az[k] = (p[k] - q[k]) * hinv;
-                }
-            }
-            float e[6];
-            e[0] = ax[0];
-            e[1] = ay[1];
-            if constexpr (DIM == 2) {
-                e[2] = 0.5f * (ax[1] + ay[0]);
-            } else {
-                e[2] = az[2];
-                e[3] = 0.5f * (ax[1] + ay[0]);
-                e[4] = 0.5f * (ax[2] + az[0]);
-                e[5] = 0.5f * (ay[2] + az[1]);
-            }
-#pragma unroll
-            for (int m = 0; m < M; ++m) ww[c][m] = w[ic + m * cs] + sigma * e[m];
-            // weighted squared norm of this channel's tensor (2D:232, 3D:582)
-            float n2 = 0.f;
-            if constexpr (DIM == 2) {
-                const float od = CPL == GR_COUPLING_UNCORRELATED ? W_OFF * ww[c][2]
-                                                                 : 2.f * ww[c][2];
-                n2 = ww[c][0] * ww[c][0] + ww[c][1] * ww[c][1] + od * od;
-            } else {
-                const float o3 = W_OFF * ww[c][3], o4 = W_OFF * ww[c][4];
-                const float o5 = CPL == GR_COUPLING_UNCORRELATED ? ww[c][5] : W_OFF * ww[c][5];
-                n2 = ww[c][0] * ww[c][0] + ww[c][1] * ww[c][1] + ww[c][2] * ww[c][2] +
-                     o3 * o3 + o4 * o4 + o5 * o5;
-            }
-            if constexpr (CPL == GR_COUPLING_UNCORRELATED) {
-                const float n = sqrtf(n2) * ainv0;
-                const float sc = n > 1.f ? 1.f / n : 1.f;
-#pragma unroll
-                for (int m = 0; m < M; ++m) ww[c][m] *= sc;
-            } else {
-                nw2 += n2;
-            }
-        }
-        if constexpr (CPL == GR_COUPLING_UNCORRELATED) {
-            float n2 = 0.f;
-#pragma unroll
-            for (int k = 0; k < D; ++k) n2 += vv[c][k] * vv[c][k];
-            const float n = sqrtf(n2) * ainv1;
-            const float sc = n > 1.f ? 1.f / n : 1.f;
-#pragma unroll
-            for (int k = 0; k < D; ++k) vv[c][k] *= sc;
-        }
+        for (int k = 0; k < D; ++k) nv2 += vv[c][k] * vv[c][k];
+        if constexpr (CPL == GR_COUPLING_UNCORRELATED)
+            project_channel<DIM, TGV>(vv[c], ww[c], n2, ainv0, ainv1);
+        else
+            nw2 += n2;
     }
 
     // coupled projections
@@ -273,20 +324,8 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
                 G[4] += vv[c][0] * vv[c][2];
                 G[5] += vv[c][1] * vv[c][2];
             }
-            float ev[3], V[3][3], S[3];
-            jacobi3<float>(G, ev, V);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float sg = ev[k] > 0.f ? sqrtf(ev[k]) : 0.f;
-                S[k] = sg * ainv1 > 1.f ? 1.f / (ainv1 * sg) : 1.f;
-            }
             float Mx[3][3];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    Mx[r][q] = S[0] * V[r][0] * V[q][0] + S[1] * V[r][1] * V[q][1] +
-                               S[2] * V[r][2] * V[q][2];
+            nuclear_matrix3(G, ainv1, Mx);
 #pragma unroll
             for (int c = 0; c < MAXC; ++c) {
                 const float a0 = vv[c][0], a1 = vv[c][1], a2 = vv[c][2];
@@ -312,6 +351,101 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
             for (int m = 0; m < M; ++m) wout[ic + m * cs] = ww[c][m] * wsc;
         }
     }
+}
+
+// Dual step at one voxel for any number of channels: the channels are swept
+// twice, once for the coupling statistics (Frobenius: the squared norms of
+// all v and w; nuclear: those and the Gram matrix of the v) and once to
+// project and write, recomputing each channel's ascent step (its inputs are
+// read twice).  Same arithmetic in the same order as dual_core; only one
+// channel is held in registers at a time.
+template <int DIM, int CPL, bool TGV, class Src, bool ZH = false>
+__device__ __forceinline__ void dual_core_wide(const ProbDev& P, int x, int y, int z,
+                                               const Src& src, const float* v, const float* w,
+                                               float* vout, float* wout)
+{
+    constexpr int D = DIM;
+    constexpr int M = DIM == 2 ? 3 : 6;
+    const int nx = P.nx, ny = P.ny, nch = P.nch;
+    const float ainv0 = P.ainv0, ainv1 = P.ainv1;
+    if (x >= nx || y >= ny) return;
+    const size_t plane = (size_t)nx * ny;
+    const size_t chs = plane * P.nz;      // channel stride
+    const size_t cs = chs * nch;          // component stride
+    const size_t i = (size_t)z * plane + (size_t)y * nx + x;
+    float vv[D];
+    float ww[TGV ? M : 1];
+    float nv2 = 0.f, nw2 = 0.f;
+    float G[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // 2D: a, b, c of the 2x2 Gram matrix
+    if constexpr (CPL != GR_COUPLING_UNCORRELATED) {
+        for (int c = 0; c < nch; ++c) {
+            nw2 += dual_candidate<DIM, CPL, TGV, Src, ZH>(P, src, v, w, x, y, z, i, c, vv, ww);
+#pragma unroll
+            for (int k = 0; k < D; ++k) nv2 += vv[k] * vv[k];
+            if constexpr (CPL == GR_COUPLING_NUCLEAR) {
+                if constexpr (DIM == 2) {
+                    G[0] += vv[0] * vv[0];
+                    G[1] += vv[0] * vv[1];
+                    G[2] += vv[1] * vv[1];
+                } else {
+                    G[0] += vv[0] * vv[0];
+                    G[1] += vv[1] * vv[1];
+                    G[2] += vv[2] * vv[2];
+                    G[3] += vv[0] * vv[1];
+                    G[4] += vv[0] * vv[2];
+                    G[5] += vv[1] * vv[2];
+                }
+            }
+        }
+    }
+    float vsc = 1.f, m00 = 1.f, m01 = 0.f, m11 = 1.f;
+    float Mx[3][3] = {{1.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, {0.f, 0.f, 1.f}};
+    if constexpr (CPL == GR_COUPLING_FROBENIUS) {
+        const float n = sqrtf(nv2) * ainv1;
+        vsc = n > 1.f ? 1.f / n : 1.f;
+    } else if constexpr (CPL == GR_COUPLING_NUCLEAR) {
+        if constexpr (DIM == 2) nucl2d(G[0], G[1], G[2], ainv1, m00, m01, m11);
+        else nuclear_matrix3(G, ainv1, Mx);
+    }
+    float wsc = 1.f;
+    if constexpr (TGV && CPL != GR_COUPLING_UNCORRELATED) {
+        const float n = sqrtf(nw2) * ainv0;
+        wsc = n > 1.f ? 1.f / n : 1.f;
+    }
+    for (int c = 0; c < nch; ++c) {
+        const float n2 = dual_candidate<DIM, CPL, TGV, Src, ZH>(P, src, v, w, x, y, z, i, c, vv, ww);
+        if constexpr (CPL == GR_COUPLING_UNCORRELATED) {
+            project_channel<DIM, TGV>(vv, ww, n2, ainv0, ainv1);
+        } else if constexpr (CPL == GR_COUPLING_FROBENIUS) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) vv[k] *= vsc;
+        } else if constexpr (DIM == 2) {
+            const float x0 = vv[0], x1 = vv[1];
+            vv[0] = x0 * m00 + x1 * m01;
+            vv[1] = x0 * m01 + x1 * m11;
+        } else {
+            const float a0 = vv[0], a1 = vv[1], a2 = vv[2];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) vv[q] = a0 * Mx[0][q] + a1 * Mx[1][q] + a2 * Mx[2][q];
+        }
+        const size_t ic = i + c * chs;
+#pragma unroll
+        for (int k = 0; k < D; ++k) vout[ic + k * cs] = vv[k];
+        if constexpr (TGV) {
+#pragma unroll
+            for (int m = 0; m < M; ++m) wout[ic + m * cs] = ww[m] * wsc;
+        }
+    }
+}
+
+template <int DIM, int CPL, bool TGV, bool ZH = false>
+__device__ __forceinline__ void dual_body_wide(const ProbDev& P, const DualItem& I)
+{
+    const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
+    const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
+    const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;
+    const MemSrc<DIM, TGV> src{P.ubar, P.pbar, cs};
+    dual_core_wide<DIM, CPL, TGV, MemSrc<DIM, TGV>, ZH>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
 }
 
 // ZH: the problem is a z-slab with halo planes (gr_tgv_state's halo_*)
@@ -357,6 +491,26 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_batch_kernel(
     const DualItem I = items[blockIdx.x];
     const ProbDev P = probs[I.prob];
     dual_body<DIM, CPL, TGV, MAXC>(P, I);
+}
+
+// More than 8 channels: the two-sweep form, one channel in registers at a time.
+template <int DIM, int CPL, bool TGV, bool ZH>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_wide_kernel(ProbDev P, int tiles_x,
+                                                                     int tiles_y)
+{
+    const int b = stencil_block<DUAL_GROUP>();
+    const int tx = b % tiles_x, rest = b / tiles_x;
+    const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
+    dual_body_wide<DIM, CPL, TGV, ZH>(P, I);
+}
+
+template <int DIM, int CPL, bool TGV>
+__global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_wide_batch_kernel(
+    const ProbDev* __restrict__ probs, const DualItem* __restrict__ items)
+{
+    const DualItem I = items[blockIdx.x];
+    const ProbDev P = probs[I.prob];
+    dual_body_wide<DIM, CPL, TGV>(P, I);
 }
 
 // ---- FP64 energy ----------------------------------------------------------
@@ -514,6 +668,20 @@ static void dual_single_launch(const ProbDev& P, hipStream_t st)
                            tiles_y);
     else
         hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, false>), dim3((unsigned)blocks),
+                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+}
+
+template <int DIM, int CPL, bool TGV>
+static void dual_wide_launch(const ProbDev& P, hipStream_t st)
+{
+    const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
+    const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
+    const long long blocks = (long long)tiles_x * tiles_y * P.nz;
+    if (DIM == 3 && (P.halo_ubar_hi || P.halo_v_lo))
+        hipLaunchKernelGGL((dual_wide_kernel<DIM, CPL, TGV, DIM == 3>), dim3((unsigned)blocks),
+                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+    else
+        hipLaunchKernelGGL((dual_wide_kernel<DIM, CPL, TGV, false>), dim3((unsigned)blocks),
                            dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
 }
 
@@ -794,7 +962,8 @@ static void dual_single_c(const ProbDev& P, hipStream_t st)
     if (P.nch <= 1) dual_single_launch<DIM, CPL, TGV, 1>(P, st);
     else if (P.nch <= 2) dual_single_launch<DIM, CPL, TGV, 2>(P, st);
     else if (P.nch <= 4) dual_single_launch<DIM, CPL, TGV, 4>(P, st);
-    else dual_single_launch<DIM, CPL, TGV, 8>(P, st);
+    else if (P.nch <= 8) dual_single_launch<DIM, CPL, TGV, 8>(P, st);
+    else dual_wide_launch<DIM, CPL, TGV>(P, st);
 }
 
 template <int DIM, int CPL>
@@ -814,10 +983,6 @@ static void dual_single_d(const ProbDev& P, hipStream_t st)
 
 int launch_dual(const ProbDev& P, hipStream_t st)
 {
-    if (P.nch > 8) {
-        set_error("dual step: %d channels exceed the 8 this build holds per voxel", P.nch);
-        return GR_ERR_UNSUPPORTED;
-    }
     const long long blocks = (long long)((P.nx + DUAL_BX - 1) / DUAL_BX) *
                              ((P.ny + DUAL_BY - 1) / DUAL_BY) * P.nz;
     if (blocks > 0x7fffffff) {
@@ -834,15 +999,16 @@ int launch_dual_batch(const ProbDev* probs, const DualItem* items, int nitems, i
                       int coupling, bool tgv, int maxc, hipStream_t st)
 {
     if (nitems <= 0) return GR_OK;
-    if (maxc > 8) {
-        set_error("dual step: more than 8 channels per voxel");
-        return GR_ERR_UNSUPPORTED;
-    }
     dim3 grd((unsigned)nitems), blk(DUAL_BX * DUAL_BY);
 #define GR_DB(DD, CC, TT, MM) \
     hipLaunchKernelGGL((dual_batch_kernel<DD, CC, TT, MM>), grd, blk, 0, st, probs, items)
-#define GR_DB_M(DD, CC, TT) \
-    do { if (maxc <= 4) GR_DB(DD, CC, TT, 4); else GR_DB(DD, CC, TT, 8); } while (0)
+#define GR_DB_M(DD, CC, TT)                                                                 \
+    do {                                                                                    \
+        if (maxc <= 4) GR_DB(DD, CC, TT, 4);                                                \
+        else if (maxc <= 8) GR_DB(DD, CC, TT, 8);                                           \
+        else hipLaunchKernelGGL((dual_wide_batch_kernel<DD, CC, TT>), grd, blk, 0, st, probs, \
+                                items);                                                     \
+    } while (0)
 #define GR_DB_T(DD, CC) \
     do { if (tgv) GR_DB_M(DD, CC, true); else GR_DB_M(DD, CC, false); } while (0)
 #define GR_DB_C(DD)                                                        \

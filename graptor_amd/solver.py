@@ -90,7 +90,7 @@ class TgvProblem:
                ~10 more floats of memory per voxel-channel).  Off by default:
                the fused kernel re-evaluates the primal step at the six
                neighbours it needs and measured 1.47 ms against 0.70 ms for
-               the two separate kernels on 256x256x64x4.
+               the two separate kernels on 256x256x64x4.  At most 8 channels.
     """
 
     def __init__(self, f, geometry: RadonGeometry, alpha, mu, dim=3,
@@ -160,6 +160,8 @@ class TgvProblem:
         # backprojection
         self.ktl = None if getattr(self.geometry, "fuse_primal", False) else z(*img)
         # second state buffers of the fused primal + next-dual kernel
+        if self.fused and self.C > 8:
+            raise ValueError("fused=True: the fused kernel holds at most 8 channels per voxel")
         alt = self.fused and self.ktl is not None
         self.u2 = z(*img) if alt else None
         self.v2 = z(d, *img) if alt else None

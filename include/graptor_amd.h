@@ -110,7 +110,10 @@ typedef struct gr_tgv_params {
  * TV), gr_tgv_iterate runs the primal step of each iteration and the dual
  * step of the next as one kernel, alternating the state between the two
  * sets (pbar serves as the second p); the state ends in u, p, v, w as
- * without them and the values are the same. */
+ * without them and the values are the same.  That fused form holds at most 8
+ * channels (more give GR_ERR_UNSUPPORTED); every other path takes any
+ * channel count (the dual step holds up to 8 channels of a voxel in
+ * registers and sweeps larger counts twice). */
 typedef struct gr_tgv_state {
     float *u, *ubar, *p, *pbar, *v, *w;
     float *lam, *ktl, *ku;
@@ -206,8 +209,8 @@ int gr_tgv_energy_noalloc(const gr_geometry *g, const gr_tgv_params *prm,
  * coupling, discrepancy and state, advanced together with one launch per
  * kernel class and phase (at most a few launches per phase for the whole
  * batch).  The plan keeps device copies of the descriptors; the state
- * buffers stay caller-owned and must outlive the plan.  A problem with more
- * than 8 channels or with z halo planes gives GR_ERR_UNSUPPORTED. */
+ * buffers stay caller-owned and must outlive the plan.  A problem with z
+ * halo planes gives GR_ERR_UNSUPPORTED. */
 typedef struct gr_plan gr_plan;
 int gr_plan_create(int nprob, const gr_geometry *const *geoms,
                    const gr_tgv_params *prms, const gr_tgv_state *states,

@@ -22,8 +22,10 @@ SPECS = [
     (3, 36, 20, 2, 5, "frobenius", "KL"),
     (2, 44, 28, 6, 1, "frobenius", "KL"),       # 5..8 channels: the 8-channel kernels
     (3, 32, 18, 8, 3, "uncorrelated", "L2"),
+    (2, 36, 24, 10, 1, "nuclear", "KL"),        # more than 8 channels: the two-sweep kernels
+    (3, 28, 16, 9, 2, "frobenius", "KL"),
 ]
-MU = [1.0, 0.3, 0.2, 0.1, 0.5, 0.4, 0.25, 0.15, 0.35]
+MU = [1.0, 0.3, 0.2, 0.1, 0.5, 0.4, 0.25, 0.15, 0.35, 0.45]
 
 
 def _make(seed, spec):
@@ -78,14 +80,15 @@ def test_batch_matches_individual_solves():
         assert eu <= 1e-4 and ek <= 1e-4
 
 
-def test_plan_rejects_more_than_eight_channels():
-    """The batched dual kernel holds at most 8 channels of a voxel: a plan
-    with a 9-channel problem is refused instead of dropping channels."""
-    from graptor_amd.batch import BatchPlan
-    ok = _make(480, SPECS[0])[0]
-    nine = _make(481, (2, 32, 16, 9, 1, "frobenius", "KL"))[0]
-    with pytest.raises(NotImplementedError, match="at most 8"):
-        BatchPlan([ok, nine])
+def test_fused_form_refuses_more_than_eight_channels():
+    """The fused primal + next-dual kernel holds at most 8 channels of a
+    voxel; every other path (the batches above included) takes any count."""
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    _, f, ofs = make_problem(481, 9, 1, 32, 16, poisson=20.0)
+    g = RadonGeometry((32, 32), angles_deg(-70, 70, 16), n_detectors=32)
+    with pytest.raises(ValueError, match="at most 8"):
+        TgvProblem(f.astype(np.float32), g, (4.0, 1.0), MU[:9], dim=2, L=10.0, fused=True)
 
 
 def test_graph_replay_matches_eager():

@@ -30,20 +30,23 @@ torch = pytest.importorskip("torch")
 
 COMBOS = list(itertools.product((2, 3), O.COUPLINGS, ("TGV", "TV"), ("KL", "L2")))
 IDS = ["-".join(map(str, c)) for c in COMBOS]
+# more than 8 channels: the dual step's two-sweep form
+WIDE = list(itertools.product((2, 3), O.COUPLINGS, ("TGV", "TV")))
+WIDE_IDS = ["-".join(map(str, c)) for c in WIDE]
 BAR = 1e-5
 
 
-def _setup(dim, coupling, reg, disc, seed=90):
+def _setup(dim, coupling, reg, disc, seed=90, C=3):
     """A TgvProblem whose state holds random fields; alpha is chosen so that
     the scaled radii alpha/L^2 are (0.8, 0.5) and the fields straddle them."""
     from graptor_amd.projector import RadonGeometry
     from graptor_amd.solver import TgvProblem
-    C, Nz, N, na = 3, (1 if dim == 2 else 4), 40, 20
+    Nz, N, na = (1 if dim == 2 else 4), 40, 20
     _, f, ofs = make_problem(seed, C, Nz, N, na, poisson=40.0)
     g = RadonGeometry((N, N), angles_deg(-70, 70, na), n_detectors=N)
     L = O.normest_power(ofs, N, N, N)
     alpha = (0.8 * L * L, 0.5 * L * L)
-    mu = [1.0, 0.4, 0.2]
+    mu = [1.0, 0.4, 0.2] if C == 3 else list(np.linspace(1.0, 0.1, C))
     prob = TgvProblem(f.astype(np.float32), g, alpha, mu, dim=dim, coupling=coupling,
                       discrepancy=disc, regularisation=reg, stepsize=1.5 if dim == 3 else 1.0,
                       L=L)
@@ -112,11 +115,10 @@ def _check(name, got, want, tag):
     assert err <= BAR, (tag, name, err)
 
 
-@pytest.mark.parametrize("combo", COMBOS, ids=IDS)
-def test_one_iteration_operator_by_operator(combo):
+def _one_iteration(combo, C=3):
     dim, cpl, reg, disc = combo
-    print(f"one iteration {combo}", flush=True)
-    prob, f, ofs = _setup(dim, cpl, reg, disc)
+    print(f"one iteration {combo}, {C} channels", flush=True)
+    prob, f, ofs = _setup(dim, cpl, reg, disc, C=C)
     s = _host(prob)
     k = _consts(prob)
     prob.iterate(1)
@@ -137,6 +139,16 @@ def test_one_iteration_operator_by_operator(combo):
     if pn is not None:
         _check("p", g["p"], pn, "primal")
         _check("pbar", g["pbar"], 2 * pn - s["p"], "primal")
+
+
+@pytest.mark.parametrize("combo", COMBOS, ids=IDS)
+def test_one_iteration_operator_by_operator(combo):
+    _one_iteration(combo)
+
+
+@pytest.mark.parametrize("combo", WIDE, ids=WIDE_IDS)
+def test_one_iteration_more_than_eight_channels(combo):
+    _one_iteration(combo + ("KL",), C=11)
 
 
 @pytest.mark.parametrize("combo", COMBOS, ids=IDS)
