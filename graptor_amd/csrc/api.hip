@@ -610,7 +610,10 @@ int gr_geometry_create(const float* ofs, int na, int nx, int ny, int nd, gr_geom
     Geometry& g = gg->g;
     g.nx = nx; g.ny = ny; g.nd = nd; g.na = na;
     g.fwd_band = 0;
-    g.adj_tile_y = 32;
+    // backprojection tiles: 16 rows up to 256 px (more work-groups to share the
+    // GPU with the dual step: 0.801 against 0.810 ms per two-stream iteration
+    // on 256x256x64x4), 32 above (18.54 against 18.69 ms on 512x512x256x4)
+    g.adj_tile_y = std::max(nx, ny) <= 256 ? 16 : 32;
     g.fuse_primal = 0;
     g.ofs_dev = nullptr; g.fwd_dev = nullptr; g.chunks_dev = nullptr;
     std::vector<FwdAngle> fa(na);

@@ -55,7 +55,9 @@ typedef struct gr_geometry gr_geometry;
 int gr_geometry_create(const float *ofs, int na, int nx, int ny, int nd,
                        gr_geometry **out);
 int gr_geometry_destroy(gr_geometry *g);
-/* Tile shapes of the projector kernels (autotuner hook); 0 keeps a value. */
+/* Tile shapes of the projector kernels (autotuner hook); 0 keeps a value.
+ * Defaults: forward bands sized automatically; backprojection tiles of 16
+ * rows for images up to 256 px, 32 rows above. */
 int gr_geometry_set_tuning(gr_geometry *g, int fwd_band, int adj_tile_y);
 int gr_geometry_get_tuning(const gr_geometry *g, int *fwd_band, int *adj_tile_y);
 /* 1: run the primal step inside the backprojection kernel (K^T lam never
