@@ -99,8 +99,10 @@ struct Geometry {
 // Modes of the forward projector epilogue.
 // FWD_PROX applies each problem's own discrepancy (batches).
 // FWD_BESIDE: the launch shares the GPU with the dual step of the same
-// iteration (two-stream schedule): fewer, larger work-groups
-enum { FWD_WRITE = 1, FWD_KL = 2, FWD_L2 = 4, FWD_PROX = 8, FWD_BESIDE = 16 };
+// iteration (two-stream schedule): fewer, larger work-groups.
+// FWD_AUTOBAND: the geometry's band height is automatic, so a launch beside
+// the dual step may size it for its grid.
+enum { FWD_WRITE = 1, FWD_KL = 2, FWD_L2 = 4, FWD_PROX = 8, FWD_BESIDE = 16, FWD_AUTOBAND = 32 };
 
 constexpr int FWD_THREADS = 256;
 constexpr int FWD_PADL = 4;          // zero candidates left of position 0
@@ -146,7 +148,7 @@ __device__ __forceinline__ int stencil_block()
 // Kernel-side sizes derived from a problem and its tuning.
 int fwd_pitch_of(int nx, int ny);
 int fwd_table_choice(const Geometry& g, long long groups, double wg_per_cu = 3.0);
-int fwd_band_auto(int pitch, int K);
+int fwd_band_auto(int pitch, int K, int lds_kb = 36);
 int adj_segw_of(int tile_y);
 int adj_achunk_of(int na, int segw, int K);
 size_t fwd_lds_bytes(const ProbDev& P, int K);

@@ -487,10 +487,10 @@ int fwd_pitch_of(int nx, int ny)
     return p;
 }
 
-int fwd_band_auto(int pitch, int K)
+int fwd_band_auto(int pitch, int K, int lds_kb)
 {
-    // about 36 KB of LDS per work-group: four work-groups per CU
-    int b = (int)(36 * 1024 / ((size_t)pitch * K * sizeof(float)));
+    // lines per band within lds_kb of LDS per work-group (36 KB: four per CU)
+    int b = (int)((size_t)lds_kb * 1024 / ((size_t)pitch * K * sizeof(float)));
     return std::max(4, std::min(32, b & ~3));
 }
 
@@ -568,6 +568,17 @@ static int fwd_single_k(const ProbDev& Pin, const Geometry& g, int nslices, int 
     const FwdChunk* chunks = g.chunks_dev + g.tab_off[t];
     P.fwd_pitch = fwd_pitch_of(P.nx, P.ny);
     if (P.fwd_band <= 0) P.fwd_band = fwd_band_auto(P.fwd_pitch, K);
+    if ((mode & FWD_BESIDE) && (mode & FWD_AUTOBAND)) {
+        // beside the dual step, bands as tall as an even spread of the grid
+        // over the 256 CUs allows: the LDS of a work-group then caps the
+        // work-groups a CU takes, leaving every CU room for the dual step
+        // (256^2 images: 512 work-groups, 2 per CU, 16-line bands: 0.786
+        // against 0.804 ms per iteration with 8; 512^2: grids of thousands
+        // keep 4 per CU and 4 lines, 18.6 against 19.2 ms with 8)
+        const long long per_cu =
+            std::min(4LL, std::max(1LL, ((long long)groups * nchunks + 255) / 256));
+        P.fwd_band = fwd_band_auto(P.fwd_pitch, K, (int)(160 / per_cu) - 1);
+    }
     const size_t lds = fwd_lds_bytes(P, K);
     if (lds > 160 * 1024) {
         set_error("radon: band %d does not fit LDS (%zu bytes)", P.fwd_band, lds);
