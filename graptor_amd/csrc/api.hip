@@ -261,7 +261,7 @@ int iterate_single(const gr_geometry* gg, const ProbDev& P, int n_iter, int ku_o
             rc = launch_fwd(P, gg->g, nsl,
                             fwd_mode(it) | FWD_BESIDE | (gg->g.fwd_band > 0 ? 0 : FWD_AUTOBAND),
                             st);
-        if (!rc) rc = launch_adj(P, nsl, false, st);
+        if (!rc) rc = launch_adj(P, nsl, false, st, true);
         if (!rc) rc = hip_rc(hipStreamWaitEvent(st, lane->dual_done, 0), "dual wait");
         if (!rc) rc = launch_primal_part(P, PRIMAL_U, st);
     }

@@ -158,8 +158,9 @@ int slice_group(int nslices);  // K = 4 (3 or more slices), 2 or 1 slices sharin
 // ---- launchers (single problem: descriptor by value) ---------------------
 int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipStream_t st);
 // adjoint only (primal == false: writes K^T lam into P.ktl) or fused with
-// the primal step of the iteration (primal == true)
-int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st);
+// the primal step of the iteration (primal == true); beside: the launch
+// shares the GPU with the dual step (two-stream schedule)
+int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st, bool beside = false);
 int launch_dual(const ProbDev& P, hipStream_t st);
 int launch_fused_pd(const ProbDev& P, const FusedBufs& B, hipStream_t st);
 int launch_primal(const ProbDev& P, hipStream_t st);

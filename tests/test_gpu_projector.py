@@ -119,13 +119,15 @@ def test_tile_shapes_agree(band, tile):
     assert rel(back.cpu().numpy(), O.radon_adj(s64, ofs, 100, 100)) <= 1e-5
 
 
-@pytest.mark.parametrize("nslices", [1, 2, 3])
-def test_short_tiles_every_slice_group(nslices):
+@pytest.mark.parametrize("tile", [16, 32])
+@pytest.mark.parametrize("nslices", [1, 2, 3, 8, 11])
+def test_tiles_every_slice_group(nslices, tile):
     """Tiles of <= 16 rows take the backprojection with 4 accumulator rows
-    per lane; slice groups of 1, 2 and 4 (3 slices: one padded group)."""
+    per lane, 32-row tiles with 8; slice groups of 1, 2, 4 (3 slices: one
+    padded group) and 8 (two LDS planes; 11 slices: one padded group)."""
     from graptor_amd import radon_ad
     g = _geom(70, 70, 80, angles_deg(-70, 70, 19))
-    g.set_tuning(fwd_band=8, adj_tile_y=16)
+    g.set_tuning(fwd_band=8, adj_tile_y=tile)
     rng = np.random.default_rng(12 + nslices)
     sino = rng.random((nslices, 19, 80)).astype(np.float32)
     back = torch.zeros((nslices, 70, 70), device="cuda")
