@@ -703,10 +703,13 @@ int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st, bool 
 {
     if (nslices <= 0) return GR_OK;
     // eight slices per work-group: each interpolation weight serves twice the
-    // multiply-adds (256^2 x 256 slices, 60 angles: 0.111 against 0.131 ms
-    // alone, same bits).  Beside the dual step four (0.7860 against 0.7874 ms
+    // multiply-adds (256^2 x 256 slices, 60 angles, 16-row tiles: 0.111
+    // against 0.131 ms alone, same bits).  Not with 32-row tiles (8 rows x 8
+    // slices of accumulators per lane: 2.92 against 2.47 ms on 512^2 x 1024
+    // slices, 90 angles), nor beside the dual step (0.7874 against 0.7860 ms
     // per two-stream iteration).
-    if (nslices >= 8 && !primal && !beside) return adj_single_k<8>(P, nslices, primal, st);
+    if (nslices >= 8 && !primal && !beside && P.adj_tile_y <= 16)
+        return adj_single_k<8>(P, nslices, primal, st);
     switch (slice_group(nslices)) {
     case 4: return adj_single_k<4>(P, nslices, primal, st);
     case 2: return adj_single_k<2>(P, nslices, primal, st);
