@@ -280,15 +280,16 @@ __device__ __forceinline__ void primal_update_r(
     float un, ub, pn[3], pb[3];
     primal_point<DIM, TGV, NU, NP, true, ZH>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
                                    z, un, ub, pn, pb, zh);
+    // streamed once per iteration: non-temporal stores (as the dual step's)
     if constexpr (NU) {
-        u[i] = un;
-        ubar[i] = ub;
+        __builtin_nontemporal_store(un, &u[i]);
+        __builtin_nontemporal_store(ub, &ubar[i]);
     }
     if constexpr (TGV && NP) {
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            p[i + k * cs] = pn[k];
-            pbar[i + k * cs] = pb[k];
+            __builtin_nontemporal_store(pn[k], &p[i + k * cs]);
+            __builtin_nontemporal_store(pb[k], &pbar[i + k * cs]);
         }
     }
 }
@@ -312,7 +313,7 @@ __device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
     const int s = I.z;  // slice z + nz * c
     if (x >= P.nx || y >= P.ny) return;
     const size_t i = (size_t)s * P.nx * P.ny + (size_t)y * P.nx + x;
-    const float ktl = (PART & PRIMAL_U) ? P.ktl[i] : 0.f;
+    const float ktl = (PART & PRIMAL_U) ? __builtin_nontemporal_load(&P.ktl[i]) : 0.f;
     primal_update<DIM, TGV, PART, ZH>(P, i, x, y, s % P.nz, ktl);
 }
 
@@ -346,7 +347,8 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_u_voxel_kernel(ProbDe
     const size_t plane = (size_t)P.nx * P.ny;
     for (int c = 0; c < P.nch; ++c) {
         const size_t i = (size_t)(z + P.nz * c) * plane + (size_t)y * P.nx + x;
-        primal_update<DIM, TGV, PRIMAL_U>(P, i, x, y, z, P.ktl[i]);
+        const float ktl = __builtin_nontemporal_load(&P.ktl[i]);
+        primal_update<DIM, TGV, PRIMAL_U>(P, i, x, y, z, ktl);
     }
 }
 

@@ -78,7 +78,7 @@ __device__ __forceinline__ void primal_point(
             }
             dv += d2 * hinv;
         }
-        const float uo = u[i];
+        const float uo = __builtin_nontemporal_load(&u[i]);  // own voxel only
         const float t = uo + tau * (dv - norming * ktl);
         un = t < 0.f ? 0.f : t;
         ub = 2.f * un - uo;
@@ -132,7 +132,7 @@ __device__ __forceinline__ void primal_point(
         }
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            const float po = p[i + k * cs];
+            const float po = __builtin_nontemporal_load(&p[i + k * cs]);
             pn[k] = po + tau * (vc[k] + q[k]);
             pb[k] = 2.f * pn[k] - po;
         }
