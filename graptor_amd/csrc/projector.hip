@@ -269,7 +269,7 @@ __global__ __launch_bounds__(FWD_THREADS) void fwd_batch_kernel(const ProbDev* _
 //   p+ = p + tau (v + div2 w),               pbar = 2 p+ - p
 // i indexes u[c][z][y][x] (= slice s = z + nz c), z = s % nz.
 // --------------------------------------------------------------------------
-template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false>
+template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void primal_update_r(
     float* __restrict__ u, float* __restrict__ ubar, float* __restrict__ p,
     float* __restrict__ pbar, const float* __restrict__ v, const float* __restrict__ w, int nx,
@@ -278,34 +278,34 @@ __device__ __forceinline__ void primal_update_r(
 {
     constexpr bool NU = (PART & PRIMAL_U) != 0, NP = (PART & PRIMAL_P) != 0;
     float un, ub, pn[3], pb[3];
-    primal_point<DIM, TGV, NU, NP, true, ZH>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
+    primal_point<DIM, TGV, NU, NP, true, ZH, NT>(u, p, v, w, ktl, nx, ny, nz, cs, hinv, tau, norming, i, x, y,
                                    z, un, ub, pn, pb, zh);
-    // streamed once per iteration: non-temporal stores (as the dual step's)
+    // streamed once per iteration (NT: with the non-temporal hint)
     if constexpr (NU) {
-        __builtin_nontemporal_store(un, &u[i]);
-        __builtin_nontemporal_store(ub, &ubar[i]);
+        st_stream<NT>(un, &u[i]);
+        st_stream<NT>(ub, &ubar[i]);
     }
     if constexpr (TGV && NP) {
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            __builtin_nontemporal_store(pn[k], &p[i + k * cs]);
-            __builtin_nontemporal_store(pb[k], &pbar[i + k * cs]);
+            st_stream<NT>(pn[k], &p[i + k * cs]);
+            st_stream<NT>(pb[k], &pbar[i + k * cs]);
         }
     }
 }
 
-template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false>
+template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void primal_update(const ProbDev& P, size_t i, int x, int y, int z,
                                               float ktl)
 {
     const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;  // component stride
-    primal_update_r<DIM, TGV, PART, ZH>(P.u, P.ubar, P.p, P.pbar, P.v, P.w, P.nx, P.ny, P.nz, cs, P.hinv,
-                              P.tau, P.norming, i, x, y, z, ktl, &P);
+    primal_update_r<DIM, TGV, PART, ZH, NT>(P.u, P.ubar, P.p, P.pbar, P.v, P.w, P.nx, P.ny, P.nz,
+                                            cs, P.hinv, P.tau, P.norming, i, x, y, z, ktl, &P);
 }
 
 // Primal step of one voxel-channel per thread with K^T lam read from memory
 // (the unfused form: high occupancy for the memory-bound update).
-template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false>
+template <int DIM, bool TGV, int PART = PRIMAL_UP, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
 {
     const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
@@ -313,8 +313,8 @@ __device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
     const int s = I.z;  // slice z + nz * c
     if (x >= P.nx || y >= P.ny) return;
     const size_t i = (size_t)s * P.nx * P.ny + (size_t)y * P.nx + x;
-    const float ktl = (PART & PRIMAL_U) ? __builtin_nontemporal_load(&P.ktl[i]) : 0.f;
-    primal_update<DIM, TGV, PART, ZH>(P, i, x, y, s % P.nz, ktl);
+    const float ktl = (PART & PRIMAL_U) ? ld_stream<NT>(&P.ktl[i]) : 0.f;
+    primal_update<DIM, TGV, PART, ZH, NT>(P, i, x, y, s % P.nz, ktl);
 }
 
 // PART selects the u half (needs K^T lam), the p half (needs only v and w,
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_single_kernel(ProbDev
     const int b = stencil_block<PRIMAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    primal_body<DIM, TGV, PART, ZH>(P, I);
+    primal_body<DIM, TGV, PART, ZH, true>(P, I);
 }
 
 // The u half with all channels of a voxel in one thread (a quarter of the
@@ -347,8 +347,8 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_u_voxel_kernel(ProbDe
     const size_t plane = (size_t)P.nx * P.ny;
     for (int c = 0; c < P.nch; ++c) {
         const size_t i = (size_t)(z + P.nz * c) * plane + (size_t)y * P.nx + x;
-        const float ktl = __builtin_nontemporal_load(&P.ktl[i]);
-        primal_update<DIM, TGV, PRIMAL_U>(P, i, x, y, z, ktl);
+        const float ktl = ld_stream<true>(&P.ktl[i]);
+        primal_update<DIM, TGV, PRIMAL_U, false, true>(P, i, x, y, z, ktl);
     }
 }
 

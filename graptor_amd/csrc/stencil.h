@@ -35,7 +35,8 @@ __device__ __forceinline__ float lane_next(float v)
 // (lane = x - x0, every lane below the last active one active), so the x - 1
 // and x + 1 neighbours of v and w come from the neighbouring lanes and only
 // the wave's edge lanes read them from memory.
-template <int DIM, bool TGV, bool NEED_U, bool NEED_P, bool LANEX = false, bool ZH = false>
+template <int DIM, bool TGV, bool NEED_U, bool NEED_P, bool LANEX = false, bool ZH = false,
+          bool NT = false>
 __device__ __forceinline__ void primal_point(
     const float* __restrict__ u, const float* __restrict__ p, const float* __restrict__ v,
     const float* __restrict__ w, float ktl, int nx, int ny, int nz, size_t cs, float hinv,
@@ -78,7 +79,7 @@ __device__ __forceinline__ void primal_point(
             }
             dv += d2 * hinv;
         }
-        const float uo = __builtin_nontemporal_load(&u[i]);  // own voxel only
+        const float uo = ld_stream<NT>(&u[i]);  // own voxel only
         const float t = uo + tau * (dv - norming * ktl);
         un = t < 0.f ? 0.f : t;
         ub = 2.f * un - uo;
@@ -132,7 +133,7 @@ __device__ __forceinline__ void primal_point(
         }
 #pragma unroll
         for (int k = 0; k < DIM; ++k) {
-            const float po = __builtin_nontemporal_load(&p[i + k * cs]);
+            const float po = ld_stream<NT>(&p[i + k * cs]);
             pn[k] = po + tau * (vc[k] + q[k]);
             pb[k] = 2.f * pn[k] - po;
         }

@@ -97,14 +97,15 @@ struct FusedSrc {
 // One channel's dual ascent at one voxel, before any projection:
 // vv = v + sigma (grad ubar - pbar), ww = w + sigma E(pbar).  Returns the
 // weighted squared norm of ww (2D:232, 3D:582; 0 without TGV).  v and w are
-// read at the voxel itself only, so their loads (and the stores of the new
-// v and w) carry the non-temporal hint: streamed once, they would otherwise
-// displace the ubar / pbar lines that the neighbouring voxels read again
-// (dual step 0.309 -> 0.287 ms, two-stream iteration 0.776 -> 0.735 ms on
-// 256x256x64x4 together with the primal step's hints, same bits); ubar and
-// pbar are read at the neighbours too (x neighbours from the adjacent
-// lanes of a 64-wide row) and keep ordinary loads.
-template <int DIM, int CPL, bool TGV, class Src, bool ZH>
+// read at the voxel itself only, so with NT (single problems) their loads
+// (and the stores of the new v and w) carry the non-temporal hint: streamed
+// once, they would otherwise displace the ubar / pbar lines that the
+// neighbouring voxels read again (dual step 0.309 -> 0.287 ms, two-stream
+// iteration 0.776 -> 0.735 ms on 256x256x64x4 together with the primal
+// step's hints, same bits); ubar and pbar are read at the neighbours too (x
+// neighbours from the adjacent lanes of a 64-wide row) and keep ordinary
+// loads.
+template <int DIM, int CPL, bool TGV, class Src, bool ZH, bool NT>
 __device__ __forceinline__ float dual_candidate(const ProbDev& P, const Src& src, const float* v,
                                                 const float* w, int x, int y, int z, size_t i,
                                                 int c, float (&vv)[DIM],
@@ -144,7 +145,7 @@ __device__ __forceinline__ float dual_candidate(const ProbDev& P, const Src& src
     }
 #pragma unroll
     for (int k = 0; k < D; ++k)
-        vv[k] = __builtin_nontemporal_load(&v[ic + k * cs]) + sigma * (g[k] - p[k]);
+        vv[k] = ld_stream<NT>(&v[ic + k * cs]) + sigma * (g[k] - p[k]);
     float n2 = 0.f;
     if constexpr (TGV) {
         float ax[3] = {0.f, 0.f, 0.f}, ay[3] = {0.f, 0.f, 0.f}, az[3] = {0.f, 0.f, 0.f};
@@ -198,7 +199,7 @@ __device__ __forceinline__ float dual_candidate(const ProbDev& P, const Src& src
         }
 #pragma unroll
         for (int m = 0; m < M; ++m)
-            ww[m] = __builtin_nontemporal_load(&w[ic + m * cs]) + sigma * e[m];
+            ww[m] = ld_stream<NT>(&w[ic + m * cs]) + sigma * e[m];
         // weighted squared norm of this channel's tensor (2D:232, 3D:582)
         if constexpr (DIM == 2) {
             const float od = CPL == GR_COUPLING_UNCORRELATED ? W_OFF * ww[2] : 2.f * ww[2];
@@ -262,7 +263,7 @@ __device__ __forceinline__ void nuclear_matrix3(const float (&G)[6], float ainv1
 // sigma E pbar), written to v_out / w_out (the same arrays as v / w for the
 // standalone step: each thread reads its own entries before writing them,
 // so they are not declared restrict).
-template <int DIM, int CPL, bool TGV, int MAXC, class Src, bool ZH = false>
+template <int DIM, int CPL, bool TGV, int MAXC, class Src, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z, const Src& src,
                                           const float* v, const float* w, float* vout,
                                           float* wout)
@@ -284,7 +285,7 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
     for (int c = 0; c < MAXC; ++c) {
         if (c >= nch) break;
         const float n2 =
-            dual_candidate<DIM, CPL, TGV, Src, ZH>(P, src, v, w, x, y, z, i, c, vv[c], ww[c]);
+            dual_candidate<DIM, CPL, TGV, Src, ZH, NT>(P, src, v, w, x, y, z, i, c, vv[c], ww[c]);
 #pragma unroll
         for (int k = 0; k < D; ++k) nv2 += vv[c][k] * vv[c][k];
         if constexpr (CPL == GR_COUPLING_UNCORRELATED)
@@ -352,11 +353,11 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
         if (c >= nch) break;
         const size_t ic = i + c * chs;
 #pragma unroll
-        for (int k = 0; k < D; ++k) __builtin_nontemporal_store(vv[c][k], &vout[ic + k * cs]);
+        for (int k = 0; k < D; ++k) st_stream<NT>(vv[c][k], &vout[ic + k * cs]);
         if constexpr (TGV) {
 #pragma unroll
             for (int m = 0; m < M; ++m)
-                __builtin_nontemporal_store(ww[c][m] * wsc, &wout[ic + m * cs]);
+                st_stream<NT>(ww[c][m] * wsc, &wout[ic + m * cs]);
         }
     }
 }
@@ -367,7 +368,7 @@ __device__ __forceinline__ void dual_core(const ProbDev& P, int x, int y, int z,
 // project and write, recomputing each channel's ascent step (its inputs are
 // read twice).  Same arithmetic in the same order as dual_core; only one
 // channel is held in registers at a time.
-template <int DIM, int CPL, bool TGV, class Src, bool ZH = false>
+template <int DIM, int CPL, bool TGV, class Src, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void dual_core_wide(const ProbDev& P, int x, int y, int z,
                                                const Src& src, const float* v, const float* w,
                                                float* vout, float* wout)
@@ -387,7 +388,7 @@ __device__ __forceinline__ void dual_core_wide(const ProbDev& P, int x, int y, i
     float G[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // 2D: a, b, c of the 2x2 Gram matrix
     if constexpr (CPL != GR_COUPLING_UNCORRELATED) {
         for (int c = 0; c < nch; ++c) {
-            nw2 += dual_candidate<DIM, CPL, TGV, Src, ZH>(P, src, v, w, x, y, z, i, c, vv, ww);
+            nw2 += dual_candidate<DIM, CPL, TGV, Src, ZH, NT>(P, src, v, w, x, y, z, i, c, vv, ww);
 #pragma unroll
             for (int k = 0; k < D; ++k) nv2 += vv[k] * vv[k];
             if constexpr (CPL == GR_COUPLING_NUCLEAR) {
@@ -421,7 +422,8 @@ __device__ __forceinline__ void dual_core_wide(const ProbDev& P, int x, int y, i
         wsc = n > 1.f ? 1.f / n : 1.f;
     }
     for (int c = 0; c < nch; ++c) {
-        const float n2 = dual_candidate<DIM, CPL, TGV, Src, ZH>(P, src, v, w, x, y, z, i, c, vv, ww);
+        const float n2 =
+            dual_candidate<DIM, CPL, TGV, Src, ZH, NT>(P, src, v, w, x, y, z, i, c, vv, ww);
         if constexpr (CPL == GR_COUPLING_UNCORRELATED) {
             project_channel<DIM, TGV>(vv, ww, n2, ainv0, ainv1);
         } else if constexpr (CPL == GR_COUPLING_FROBENIUS) {
@@ -438,34 +440,36 @@ __device__ __forceinline__ void dual_core_wide(const ProbDev& P, int x, int y, i
         }
         const size_t ic = i + c * chs;
 #pragma unroll
-        for (int k = 0; k < D; ++k) __builtin_nontemporal_store(vv[k], &vout[ic + k * cs]);
+        for (int k = 0; k < D; ++k) st_stream<NT>(vv[k], &vout[ic + k * cs]);
         if constexpr (TGV) {
 #pragma unroll
             for (int m = 0; m < M; ++m)
-                __builtin_nontemporal_store(ww[m] * wsc, &wout[ic + m * cs]);
+                st_stream<NT>(ww[m] * wsc, &wout[ic + m * cs]);
         }
     }
 }
 
-template <int DIM, int CPL, bool TGV, bool ZH = false>
+template <int DIM, int CPL, bool TGV, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void dual_body_wide(const ProbDev& P, const DualItem& I)
 {
     const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
     const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
     const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;
     const MemSrc<DIM, TGV> src{P.ubar, P.pbar, cs};
-    dual_core_wide<DIM, CPL, TGV, MemSrc<DIM, TGV>, ZH>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
+    dual_core_wide<DIM, CPL, TGV, MemSrc<DIM, TGV>, ZH, NT>(P, x, y, I.z, src, P.v, P.w, P.v,
+                                                          P.w);
 }
 
 // ZH: the problem is a z-slab with halo planes (gr_tgv_state's halo_*)
-template <int DIM, int CPL, bool TGV, int MAXC, bool ZH = false>
+template <int DIM, int CPL, bool TGV, int MAXC, bool ZH = false, bool NT = false>
 __device__ __forceinline__ void dual_body(const ProbDev& P, const DualItem& I)
 {
     const int x = I.x0 + (int)(threadIdx.x & (DUAL_BX - 1));
     const int y = I.y0 + (int)(threadIdx.x / DUAL_BX);
     const size_t cs = (size_t)P.nx * P.ny * P.nz * P.nch;
     const MemSrc<DIM, TGV> src{P.ubar, P.pbar, cs};
-    dual_core<DIM, CPL, TGV, MAXC, MemSrc<DIM, TGV>, ZH>(P, x, y, I.z, src, P.v, P.w, P.v, P.w);
+    dual_core<DIM, CPL, TGV, MAXC, MemSrc<DIM, TGV>, ZH, NT>(P, x, y, I.z, src, P.v, P.w, P.v,
+                                                           P.w);
 }
 
 // Work order: x tiles fastest, then rows, then planes, with runs of
@@ -490,7 +494,7 @@ void dual_single_kernel(ProbDev P, int tiles_x, int tiles_y)
     const int b = stencil_block<DUAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    dual_body<DIM, CPL, TGV, MAXC, ZH>(P, I);
+    dual_body<DIM, CPL, TGV, MAXC, ZH, true>(P, I);
 }
 
 template <int DIM, int CPL, bool TGV, int MAXC>
@@ -510,7 +514,7 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_wide_kernel(ProbDev P, 
     const int b = stencil_block<DUAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    dual_body_wide<DIM, CPL, TGV, ZH>(P, I);
+    dual_body_wide<DIM, CPL, TGV, ZH, true>(P, I);
 }
 
 template <int DIM, int CPL, bool TGV>
