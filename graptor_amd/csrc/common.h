@@ -105,10 +105,11 @@ struct Geometry {
 enum { FWD_WRITE = 1, FWD_KL = 2, FWD_L2 = 4, FWD_PROX = 8, FWD_BESIDE = 16, FWD_AUTOBAND = 32 };
 
 // Loads / stores of arrays a stencil kernel streams once, with the
-// non-temporal hint when NT: single problems (large volumes, whose arrays
-// would otherwise displace the lines of the neighbour reads), not batches of
-// small problems, whose arrays stay in the Infinity Cache from one phase to
-// the next (the hint there: 1.135 against 1.109 ms per batch iteration).
+// non-temporal hint when NT: single problems whose state outgrows the
+// Infinity Cache (their arrays would otherwise displace the lines of the
+// neighbour reads; stream_hints decides), not problems or batches of small
+// problems whose arrays stay in that cache from one phase to the next (the
+// hint there: 1.135 against 1.109 ms per batch iteration).
 template <bool NT>
 __device__ __forceinline__ float ld_stream(const float* p)
 {
@@ -179,6 +180,15 @@ int launch_fwd(const ProbDev& P, const Geometry& g, int nslices, int mode, hipSt
 // the primal step of the iteration (primal == true); beside: the launch
 // shares the GPU with the dual step (two-stream schedule)
 int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st, bool beside = false);
+// Whether the single-problem stencil kernels stream with the non-temporal
+// hint: the stencil state (u, ubar, p, pbar, v, w, K^T lam) above
+// STREAM_HINT_MB; GRAPTOR_STREAM_HINTS = 0 or 1 forces the choice.  Per
+// two-stream iteration, 4 channels, hints off / on: 3D 256x256x8 (144 MB)
+// 0.191 / 0.207 ms, x16 (288 MB) 0.289 / 0.291, x24 (432 MB) 0.410 / 0.389,
+// x64 (1152 MB) 0.779 / 0.741; 2D 1024^2 (192 MB) 1.677 / 1.698, 2048^2
+// (768 MB) 13.57 / 13.41.
+constexpr double STREAM_HINT_MB = 320;
+bool stream_hints(const ProbDev& P);
 int launch_dual(const ProbDev& P, hipStream_t st);
 int launch_fused_pd(const ProbDev& P, const FusedBufs& B, hipStream_t st);
 int launch_primal(const ProbDev& P, hipStream_t st);

@@ -319,14 +319,14 @@ __device__ __forceinline__ void primal_body(const ProbDev& P, const DualItem& I)
 
 // PART selects the u half (needs K^T lam), the p half (needs only v and w,
 // so it can run beside the projections) or both.
-template <int DIM, bool TGV, int PART, bool ZH>
+template <int DIM, bool TGV, int PART, bool ZH, bool NT>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_single_kernel(ProbDev P, int tiles_x,
                                                                          int tiles_y)
 {
     const int b = stencil_block<PRIMAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    primal_body<DIM, TGV, PART, ZH, true>(P, I);
+    primal_body<DIM, TGV, PART, ZH, NT>(P, I);
 }
 
 // The u half with all channels of a voxel in one thread (a quarter of the
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_single_kernel(ProbDev
 // many independent load streams): 0.779 against 0.786 ms per two-stream
 // iteration on 256x256x64x4 on one box, 0.791 against 0.798 ms on another,
 // same bits.  The p half stays per channel (0.810 ms in this form).
-template <int DIM, bool TGV>
+template <int DIM, bool TGV, bool NT>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_u_voxel_kernel(ProbDev P, int tiles_x,
                                                                           int tiles_y)
 {
@@ -347,8 +347,8 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_u_voxel_kernel(ProbDe
     const size_t plane = (size_t)P.nx * P.ny;
     for (int c = 0; c < P.nch; ++c) {
         const size_t i = (size_t)(z + P.nz * c) * plane + (size_t)y * P.nx + x;
-        const float ktl = ld_stream<true>(&P.ktl[i]);
-        primal_update<DIM, TGV, PRIMAL_U, false, true>(P, i, x, y, z, ktl);
+        const float ktl = ld_stream<NT>(&P.ktl[i]);
+        primal_update<DIM, TGV, PRIMAL_U, false, NT>(P, i, x, y, z, ktl);
     }
 }
 
@@ -719,18 +719,40 @@ int launch_adj(const ProbDev& P, int nslices, bool primal, hipStream_t st, bool 
     }
 }
 
+template <int DIM, bool TGV, bool ZH, bool NT>
+static void primal_single_part_nt(const ProbDev& P, int part, dim3 grd, dim3 blk, int tiles_x,
+                                  int tiles_y, hipStream_t st)
+{
+    if (part == PRIMAL_U)
+        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_U, ZH, NT>), grd, blk, 0, st,
+                           P, tiles_x, tiles_y);
+    else if (part == PRIMAL_P)
+        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_P, ZH, NT>), grd, blk, 0, st,
+                           P, tiles_x, tiles_y);
+    else
+        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_UP, ZH, NT>), grd, blk, 0, st,
+                           P, tiles_x, tiles_y);
+}
+
 template <int DIM, bool TGV, bool ZH>
 static void primal_single_part_zh(const ProbDev& P, int part, dim3 grd, dim3 blk, int tiles_x,
                                   int tiles_y, hipStream_t st)
 {
-    if (part == PRIMAL_U)
-        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_U, ZH>), grd, blk, 0, st, P,
-                           tiles_x, tiles_y);
-    else if (part == PRIMAL_P)
-        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_P, ZH>), grd, blk, 0, st, P,
-                           tiles_x, tiles_y);
+    if (stream_hints(P))
+        primal_single_part_nt<DIM, TGV, ZH, true>(P, part, grd, blk, tiles_x, tiles_y, st);
     else
-        hipLaunchKernelGGL((primal_single_kernel<DIM, TGV, PRIMAL_UP, ZH>), grd, blk, 0, st, P,
+        primal_single_part_nt<DIM, TGV, ZH, false>(P, part, grd, blk, tiles_x, tiles_y, st);
+}
+
+template <int DIM, bool TGV>
+static void primal_u_voxel_launch(const ProbDev& P, dim3 grd, dim3 blk, int tiles_x, int tiles_y,
+                                  hipStream_t st)
+{
+    if (stream_hints(P))
+        hipLaunchKernelGGL((primal_u_voxel_kernel<DIM, TGV, true>), grd, blk, 0, st, P, tiles_x,
+                           tiles_y);
+    else
+        hipLaunchKernelGGL((primal_u_voxel_kernel<DIM, TGV, false>), grd, blk, 0, st, P,
                            tiles_x, tiles_y);
 }
 
@@ -768,15 +790,11 @@ int launch_primal_part(const ProbDev& P, int part, hipStream_t st)
         // enough work-groups with one per voxel tile: the per-voxel u half
         const dim3 gv((unsigned)voxel_blocks);
         if (P.dim == 2) {
-            if (P.tgv) hipLaunchKernelGGL((primal_u_voxel_kernel<2, true>), gv, blk, 0, st, P,
-                                   tiles_x, tiles_y);
-            else hipLaunchKernelGGL((primal_u_voxel_kernel<2, false>), gv, blk, 0, st, P,
-                                   tiles_x, tiles_y);
+            if (P.tgv) primal_u_voxel_launch<2, true>(P, gv, blk, tiles_x, tiles_y, st);
+            else primal_u_voxel_launch<2, false>(P, gv, blk, tiles_x, tiles_y, st);
         } else {
-            if (P.tgv) hipLaunchKernelGGL((primal_u_voxel_kernel<3, true>), gv, blk, 0, st, P,
-                                   tiles_x, tiles_y);
-            else hipLaunchKernelGGL((primal_u_voxel_kernel<3, false>), gv, blk, 0, st, P,
-                                   tiles_x, tiles_y);
+            if (P.tgv) primal_u_voxel_launch<3, true>(P, gv, blk, tiles_x, tiles_y, st);
+            else primal_u_voxel_launch<3, false>(P, gv, blk, tiles_x, tiles_y, st);
         }
         GR_LAUNCH_CHECK("primal_u_voxel_kernel");
         return GR_OK;

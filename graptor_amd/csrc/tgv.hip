@@ -12,6 +12,7 @@
 #include "stencil.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -486,7 +487,7 @@ constexpr int dual_min_waves()
     return (DIM == 3 && CPL == GR_COUPLING_FROBENIUS && TGV && MAXC == 4) ? 8 : 1;
 }
 
-template <int DIM, int CPL, bool TGV, int MAXC, bool ZH>
+template <int DIM, int CPL, bool TGV, int MAXC, bool ZH, bool NT>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY)
 __attribute__((amdgpu_waves_per_eu(dual_min_waves<DIM, CPL, TGV, MAXC>())))
 void dual_single_kernel(ProbDev P, int tiles_x, int tiles_y)
@@ -494,7 +495,7 @@ void dual_single_kernel(ProbDev P, int tiles_x, int tiles_y)
     const int b = stencil_block<DUAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    dual_body<DIM, CPL, TGV, MAXC, ZH, true>(P, I);
+    dual_body<DIM, CPL, TGV, MAXC, ZH, NT>(P, I);
 }
 
 template <int DIM, int CPL, bool TGV, int MAXC>
@@ -507,14 +508,14 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_batch_kernel(
 }
 
 // More than 8 channels: the two-sweep form, one channel in registers at a time.
-template <int DIM, int CPL, bool TGV, bool ZH>
+template <int DIM, int CPL, bool TGV, bool ZH, bool NT>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void dual_wide_kernel(ProbDev P, int tiles_x,
                                                                      int tiles_y)
 {
     const int b = stencil_block<DUAL_GROUP>();
     const int tx = b % tiles_x, rest = b / tiles_x;
     const DualItem I{0, tx * DUAL_BX, (rest % tiles_y) * DUAL_BY, rest / tiles_y};
-    dual_body_wide<DIM, CPL, TGV, ZH, true>(P, I);
+    dual_body_wide<DIM, CPL, TGV, ZH, NT>(P, I);
 }
 
 template <int DIM, int CPL, bool TGV>
@@ -669,18 +670,49 @@ __global__ __launch_bounds__(EN_THREADS) void data_energy_kernel(
 // --------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------
+bool stream_hints(const ProbDev& P)
+{
+    const char* e = getenv("GRAPTOR_STREAM_HINTS");
+    if (e && (e[0] == '0' || e[0] == '1')) return e[0] == '1';
+    const int D = P.dim;
+    const int per = 2 + D + 1 + (P.tgv ? 2 * D + D * (D + 1) / 2 : 0);
+    const double bytes = (double)P.nx * P.ny * P.nz * P.nch * per * sizeof(float);
+    return bytes > STREAM_HINT_MB * 1048576.0;
+}
+
+template <int DIM, int CPL, bool TGV, int MC, bool ZH>
+static void dual_single_launch_zh(const ProbDev& P, dim3 grd, int tiles_x, int tiles_y,
+                                  hipStream_t st)
+{
+    if (stream_hints(P))
+        hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, ZH, true>), grd,
+                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+    else
+        hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, ZH, false>), grd,
+                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+}
+
 template <int DIM, int CPL, bool TGV, int MC>
 static void dual_single_launch(const ProbDev& P, hipStream_t st)
 {
     const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
     const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
-    const long long blocks = (long long)tiles_x * tiles_y * P.nz;
+    const dim3 grd((unsigned)((long long)tiles_x * tiles_y * P.nz));
     if (DIM == 3 && (P.halo_ubar_hi || P.halo_v_lo))
-        hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, DIM == 3>),
-                           dim3((unsigned)blocks), dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x,
-                           tiles_y);
+        dual_single_launch_zh<DIM, CPL, TGV, MC, DIM == 3>(P, grd, tiles_x, tiles_y, st);
     else
-        hipLaunchKernelGGL((dual_single_kernel<DIM, CPL, TGV, MC, false>), dim3((unsigned)blocks),
+        dual_single_launch_zh<DIM, CPL, TGV, MC, false>(P, grd, tiles_x, tiles_y, st);
+}
+
+template <int DIM, int CPL, bool TGV, bool ZH>
+static void dual_wide_launch_zh(const ProbDev& P, dim3 grd, int tiles_x, int tiles_y,
+                                hipStream_t st)
+{
+    if (stream_hints(P))
+        hipLaunchKernelGGL((dual_wide_kernel<DIM, CPL, TGV, ZH, true>), grd,
+                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+    else
+        hipLaunchKernelGGL((dual_wide_kernel<DIM, CPL, TGV, ZH, false>), grd,
                            dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
 }
 
@@ -689,13 +721,11 @@ static void dual_wide_launch(const ProbDev& P, hipStream_t st)
 {
     const int tiles_x = (P.nx + DUAL_BX - 1) / DUAL_BX;
     const int tiles_y = (P.ny + DUAL_BY - 1) / DUAL_BY;
-    const long long blocks = (long long)tiles_x * tiles_y * P.nz;
+    const dim3 grd((unsigned)((long long)tiles_x * tiles_y * P.nz));
     if (DIM == 3 && (P.halo_ubar_hi || P.halo_v_lo))
-        hipLaunchKernelGGL((dual_wide_kernel<DIM, CPL, TGV, DIM == 3>), dim3((unsigned)blocks),
-                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+        dual_wide_launch_zh<DIM, CPL, TGV, DIM == 3>(P, grd, tiles_x, tiles_y, st);
     else
-        hipLaunchKernelGGL((dual_wide_kernel<DIM, CPL, TGV, false>), dim3((unsigned)blocks),
-                           dim3(DUAL_BX * DUAL_BY), 0, st, P, tiles_x, tiles_y);
+        dual_wide_launch_zh<DIM, CPL, TGV, false>(P, grd, tiles_x, tiles_y, st);
 }
 
 // Primal step of iteration k fused with the dual step of iteration k + 1:

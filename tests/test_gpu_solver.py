@@ -272,6 +272,45 @@ def test_two_streams_with_many_slice_groups():
         assert torch.equal(getattr(two, name), getattr(one, name)), name
 
 
+HINT_CASES = [
+    # 4096 voxel tiles: the two-stream u half takes the per-voxel kernel
+    (3, "frobenius", "KL", "TGV", 4, 256, 64),
+    (3, "nuclear", "L2", "TGV", 3, 5, 40),
+    # more than 8 channels: the two-sweep dual step
+    (2, "nuclear", "KL", "TGV", 10, 1, 48),
+    (2, "uncorrelated", "KL", "TV", 3, 1, 40),
+]
+
+
+@pytest.mark.parametrize("case", HINT_CASES, ids=["-".join(map(str, c)) for c in HINT_CASES])
+def test_stream_hints_do_not_change_iterates(case, monkeypatch):
+    """The stencil kernels with and without the non-temporal load / store
+    hints (GRAPTOR_STREAM_HINTS forces the size-based choice) give the same
+    bits, on two streams and on one."""
+    from graptor_amd.projector import RadonGeometry
+    from graptor_amd.solver import TgvProblem
+    dim, coupling, disc, reg, C, Nz, N = case
+    na = 20
+    print(f"stream hints {case}", flush=True)
+    rng = np.random.default_rng(83)
+    f = rng.poisson(20.0, size=(C, Nz, na, N)).astype(np.float32)
+    ang = np.deg2rad(np.linspace(-70, 70, na, endpoint=False)) + np.pi / 2
+    g = RadonGeometry((N, N), ang, n_detectors=N)
+    kw = dict(dim=dim, coupling=coupling, discrepancy=disc, regularisation=reg, L=float(N))
+    mu = list(np.geomspace(1.0, 1e-2, C))
+    out = {}
+    for hint in ("0", "1"):
+        monkeypatch.setenv("GRAPTOR_STREAM_HINTS", hint)
+        prob = TgvProblem(f, g, (4, 1), mu, **kw)
+        prob.iterate(9)
+        prob.profile(3)
+        torch.cuda.synchronize()
+        out[hint] = {k: getattr(prob, k).clone() for k in ("u", "p", "v", "w", "lam")
+                     if getattr(prob, k) is not None}
+    for k, t in out["0"].items():
+        assert torch.equal(t, out["1"][k]), k
+
+
 def test_two_streams_from_two_host_threads():
     """Each host thread gets its own side stream: two problems iterated from
     two threads on their own torch streams give the same bits as the same
