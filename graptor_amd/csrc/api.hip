@@ -472,13 +472,57 @@ int plan_lanes(gr_plan* pl)
     return GR_OK;
 }
 
+// Phases of a batch iteration: 0 dual step, 1 forward projection + prox,
+// 2 backprojection, 3 primal step.  The iteration runs them in three forks,
+// each an LDS / VALU-bound projection beside an HBM-bound stencil step that
+// does not touch its arrays:
+// - PHASE_DUAL_FWD: the forward projection (reads u-bar, writes lambda) and
+//   the dual step (reads u-bar / p-bar, writes v / w);
+// - PHASE_ADJ_P: the backprojection (reads lambda, writes K^T lambda; with
+//   the fused primal step the whole primal step, for problems that have no
+//   primal launch) and the p half of the primal step (reads v / w, writes
+//   p / p-bar);
+// - PHASE_U: the u half (reads v and K^T lambda, writes u / u-bar).
+constexpr int PHASE_DUAL_FWD = 4, PHASE_ADJ_P = 5, PHASE_U = 6;
+
 int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
 {
-    const size_t nl = phase == 0 ? pl->dual.size()
-                    : phase == 1 ? pl->fwd.size()
-                    : phase == 2 ? pl->adj.size()
-                                 : pl->primal.size();
+    // (phase, launch index, primal part) in launch order, with the lane of each
+    struct Job { int ph; size_t i; int part; int lane; };
+    std::vector<Job> jobs;
+    auto add = [&](int ph, size_t n) {
+        for (size_t i = 0; i < n; ++i) jobs.push_back({ph, i, PRIMAL_UP, 0});
+    };
+    if (phase == PHASE_DUAL_FWD) {
+        add(1, pl->fwd.size());
+        add(0, pl->dual.size());
+    } else if (phase == PHASE_ADJ_P) {
+        add(2, pl->adj.size());
+        for (size_t i = 0; i < pl->primal.size(); ++i)
+            if (pl->primal[i].tgv) jobs.push_back({3, i, PRIMAL_P, 0});
+    } else if (phase == PHASE_U) {
+        for (size_t i = 0; i < pl->primal.size(); ++i)
+            jobs.push_back({3, i, pl->primal[i].tgv ? PRIMAL_U : PRIMAL_UP, 0});
+    } else {
+        add(phase, phase == 0 ? pl->dual.size()
+                 : phase == 1 ? pl->fwd.size()
+                 : phase == 2 ? pl->adj.size()
+                              : pl->primal.size());
+    }
+    const size_t nl = jobs.size();
     const int lanes = (int)std::min<size_t>(nl, gr_plan::PLAN_LANES);
+    {
+        const size_t nf = phase == PHASE_DUAL_FWD ? pl->fwd.size()
+                        : phase == PHASE_ADJ_P    ? pl->adj.size()
+                                                  : nl;
+        const bool split = nf > 0 && nf < nl && (int)nl > lanes;
+        for (size_t k = 0; k < nl; ++k)
+            // more launches than lanes: the projections share all lanes but
+            // the last, and the (shorter) stencil launches queue on that one
+            jobs[k].lane = !split ? (int)(k % (size_t)std::max(lanes, 1))
+                         : k < nf  ? (int)(k % (size_t)(lanes - 1))
+                                   : lanes - 1;
+    }
     if (lanes > 1) {
         int rc = plan_lanes(pl);
         if (rc) return rc;
@@ -509,25 +553,26 @@ int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
         }
     }
     int rc = GR_OK;
-    for (size_t i = 0; i < nl && !rc; ++i) {
-        const int lane = (int)(i % (size_t)std::max(lanes, 1));
+    for (size_t k = 0; k < nl && !rc; ++k) {
+        const int lane = jobs[k].lane, ph = jobs[k].ph, part = jobs[k].part;
+        const size_t i = jobs[k].i;
         const hipStream_t s = lane == 0 ? st : pl->side[lane - 1];
-        if (phase == 0) {
+        if (ph == 0) {
             const auto& l = pl->dual[i];
             rc = launch_dual_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim, l.coupling,
                                    l.tgv != 0, l.maxc, s);
-        } else if (phase == 1) {
+        } else if (ph == 1) {
             const auto& l = pl->fwd[i];
             rc = launch_fwd_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.mi, l.lds,
                                   fwd_mode, s);
-        } else if (phase == 2) {
+        } else if (ph == 2) {
             const auto& l = pl->adj[i];
             rc = launch_adj_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.K, l.dim,
                                   l.tgv != 0, l.fuse != 0, l.lds, s);
         } else {
             const auto& l = pl->primal[i];
             rc = launch_primal_batch(pl->probs_dev, l.dev, (int)l.items.size(), l.dim,
-                                     l.tgv != 0, s);
+                                     l.tgv != 0, part, s);
         }
     }
     // join every forked lane, also after a failed launch (a capture must end joined)
@@ -543,10 +588,9 @@ int plan_iter(gr_plan* pl, int n_iter, int ku_on_last, hipStream_t st)
 {
     for (int it = 0; it < n_iter; ++it) {
         const int mode = FWD_PROX | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
-        int rc = plan_phase(pl, 0, mode, st);
-        if (!rc) rc = plan_phase(pl, 1, mode, st);
-        if (!rc) rc = plan_phase(pl, 2, mode, st);
-        if (!rc) rc = plan_phase(pl, 3, mode, st);
+        int rc = plan_phase(pl, PHASE_DUAL_FWD, mode, st);
+        if (!rc) rc = plan_phase(pl, PHASE_ADJ_P, mode, st);
+        if (!rc) rc = plan_phase(pl, PHASE_U, mode, st);
         if (rc) return rc;
     }
     return GR_OK;

@@ -202,7 +202,7 @@ int launch_fwd_batch(const ProbDev* probs_dev, const FwdItem* items, int nitems,
 int launch_adj_batch(const ProbDev* probs_dev, const AdjItem* items, int nitems, int K,
                      int dim, bool tgv, bool primal, size_t lds, hipStream_t st);
 int launch_primal_batch(const ProbDev* probs_dev, const DualItem* items, int nitems, int dim,
-                        bool tgv, hipStream_t st);
+                        bool tgv, int part, hipStream_t st);
 int launch_dual_batch(const ProbDev* probs_dev, const DualItem* items, int nitems, int dim,
                       int coupling, bool tgv, int maxc, hipStream_t st);
 

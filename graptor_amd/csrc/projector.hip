@@ -352,13 +352,13 @@ __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_u_voxel_kernel(ProbDe
     }
 }
 
-template <int DIM, bool TGV>
+template <int DIM, bool TGV, int PART>
 __global__ __launch_bounds__(DUAL_BX* DUAL_BY) void primal_batch_kernel(
     const ProbDev* __restrict__ probs, const DualItem* __restrict__ items)
 {
     const DualItem I = items[blockIdx.x];
     const ProbDev P = probs[I.prob];
-    primal_body<DIM, TGV>(P, I);
+    primal_body<DIM, TGV, PART>(P, I);
 }
 
 // --------------------------------------------------------------------------
@@ -810,17 +810,36 @@ int launch_primal_part(const ProbDev& P, int part, hipStream_t st)
     return GR_OK;
 }
 
-int launch_primal_batch(const ProbDev* probs, const DualItem* items, int nitems, int dim,
-                        bool tgv, hipStream_t st)
+template <int DIM, bool TGV>
+static void primal_batch_part(const ProbDev* probs, const DualItem* items, dim3 grd, dim3 blk,
+                              int part, hipStream_t st)
 {
-    if (nitems <= 0) return GR_OK;
+    if (part == PRIMAL_U)
+        hipLaunchKernelGGL((primal_batch_kernel<DIM, TGV, PRIMAL_U>), grd, blk, 0, st, probs,
+                           items);
+    else if (part == PRIMAL_P)
+        hipLaunchKernelGGL((primal_batch_kernel<DIM, TGV, PRIMAL_P>), grd, blk, 0, st, probs,
+                           items);
+    else
+        hipLaunchKernelGGL((primal_batch_kernel<DIM, TGV, PRIMAL_UP>), grd, blk, 0, st, probs,
+                           items);
+}
+
+int launch_primal_batch(const ProbDev* probs, const DualItem* items, int nitems, int dim,
+                        bool tgv, int part, hipStream_t st)
+{
+    if (part != PRIMAL_U && part != PRIMAL_P && part != PRIMAL_UP) {
+        set_error("primal step: invalid part %d", part);
+        return GR_ERR_INVALID;
+    }
+    if (nitems <= 0 || (part == PRIMAL_P && !tgv)) return GR_OK;  // TV: no p
     dim3 grd((unsigned)nitems), blk(DUAL_BX * DUAL_BY);
     if (dim == 2) {
-        if (tgv) hipLaunchKernelGGL((primal_batch_kernel<2, true>), grd, blk, 0, st, probs, items);
-        else hipLaunchKernelGGL((primal_batch_kernel<2, false>), grd, blk, 0, st, probs, items);
+        if (tgv) primal_batch_part<2, true>(probs, items, grd, blk, part, st);
+        else primal_batch_part<2, false>(probs, items, grd, blk, part, st);
     } else {
-        if (tgv) hipLaunchKernelGGL((primal_batch_kernel<3, true>), grd, blk, 0, st, probs, items);
-        else hipLaunchKernelGGL((primal_batch_kernel<3, false>), grd, blk, 0, st, probs, items);
+        if (tgv) primal_batch_part<3, true>(probs, items, grd, blk, part, st);
+        else primal_batch_part<3, false>(probs, items, grd, blk, part, st);
     }
     GR_LAUNCH_CHECK("primal_batch_kernel");
     return GR_OK;
