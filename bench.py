@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# FP32 vector FMA peak without packed instructions (this build compiles with
+# FP32 vector FMA peak without packed instructions (the library compiles with
 # packed FP32 off): 256 CUs x 64 FMA/clk x 2 flops x 2.4 GHz
 VALU_PEAK_TFLOPS = 78.6
 ALPHA = (4.0, 1.0)
@@ -587,7 +587,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU transport "
+        print("bench.py measures one GPU only: the library has no multi-GPU transport "
               "(one rank's z-slab with halo planes: --workload slab); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)

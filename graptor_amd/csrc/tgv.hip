@@ -984,7 +984,7 @@ static void fused_d(const ProbDev& P, const FusedBufs& B, hipStream_t st)
 int launch_fused_pd(const ProbDev& P, const FusedBufs& B, hipStream_t st)
 {
     if (P.nch > 8) {
-        set_error("fused step: %d channels exceed the 8 this build holds per voxel", P.nch);
+        set_error("fused step: %d channels exceed the 8 the fused kernel holds per voxel", P.nch);
         return GR_ERR_UNSUPPORTED;
     }
     const long long blocks = (long long)((P.nx + DUAL_BX - 1) / DUAL_BX) *

@@ -29,7 +29,7 @@ enum {
     GR_ERR_INVALID = 1,     /* bad argument or shape */
     GR_ERR_OOM = 2,         /* device allocation failed (maps to MemoryError) */
     GR_ERR_HIP = 3,         /* any other HIP runtime failure */
-    GR_ERR_UNSUPPORTED = 4  /* valid request this build does not implement */
+    GR_ERR_UNSUPPORTED = 4  /* valid request the library does not implement */
 };
 
 enum { GR_COUPLING_UNCORRELATED = 0, GR_COUPLING_FROBENIUS = 1, GR_COUPLING_NUCLEAR = 2 };
