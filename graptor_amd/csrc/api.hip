@@ -584,13 +584,28 @@ int plan_phase(gr_plan* pl, int phase, int fwd_mode, hipStream_t st)
     return rc;
 }
 
+// Below this many voxel-channels a batch iteration keeps the four phases in
+// sequence: its kernels are latency-bound, and the extra fork and launch of
+// the three-fork order cost more than the overlap gains (183x183, one
+// channel, graph replay: 0.0887 against 0.0821 ms per iteration; 64 2D
+// problems, 11.5 M voxel-channels: 1.033 against 1.108 ms).
+constexpr double PLAN_OVERLAP_MIN = 1 << 20;
+
 int plan_iter(gr_plan* pl, int n_iter, int ku_on_last, hipStream_t st)
 {
+    double vc = 0.0;
+    for (const auto& P : pl->probs) vc += (double)P.nx * P.ny * P.nz * P.nch;
+    const bool overlap = vc >= PLAN_OVERLAP_MIN;
     for (int it = 0; it < n_iter; ++it) {
         const int mode = FWD_PROX | ((ku_on_last && it == n_iter - 1) ? FWD_WRITE : 0);
-        int rc = plan_phase(pl, PHASE_DUAL_FWD, mode, st);
-        if (!rc) rc = plan_phase(pl, PHASE_ADJ_P, mode, st);
-        if (!rc) rc = plan_phase(pl, PHASE_U, mode, st);
+        int rc = GR_OK;
+        if (overlap) {
+            rc = plan_phase(pl, PHASE_DUAL_FWD, mode, st);
+            if (!rc) rc = plan_phase(pl, PHASE_ADJ_P, mode, st);
+            if (!rc) rc = plan_phase(pl, PHASE_U, mode, st);
+        } else {
+            for (int ph = 0; ph < 4 && !rc; ++ph) rc = plan_phase(pl, ph, mode, st);
+        }
         if (rc) return rc;
     }
     return GR_OK;

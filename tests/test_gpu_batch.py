@@ -169,9 +169,10 @@ def test_autotune_keeps_results():
 def test_batch_mixes_tv_and_fused_primal():
     """A plan whose problems take different primal-step routes: TV (no p
     half), the primal step fused into the backprojection (no primal launch),
-    2D and 3D TGV.  The p half of the primal step runs beside the
-    backprojection and the dual step beside the forward projection; eager and
-    graph-replayed iterations must reproduce each problem solved on its own."""
+    2D and 3D TGV, large enough in total (over 2^20 voxel-channels) that the
+    p half of the primal step runs beside the backprojection and the dual
+    step beside the forward projection; eager and graph-replayed iterations
+    must reproduce each problem solved on its own."""
     from graptor_amd.batch import BatchPlan
     from graptor_amd.projector import RadonGeometry
     from graptor_amd.solver import TgvProblem
@@ -182,6 +183,9 @@ def test_batch_mixes_tv_and_fused_primal():
         (3, 32, 18, 2, 4, "nuclear", "L2", "TGV", False),
         (2, 36, 22, 1, 1, "uncorrelated", "L2", "TGV", False),
         (3, 28, 16, 3, 3, "uncorrelated", "KL", "TV", True),
+        # 2^20 voxel-channels: the plan's total reaches the size from which it
+        # overlaps projections and stencil steps
+        (3, 128, 24, 2, 32, "frobenius", "KL", "TGV", False),
     ]
 
     def make(seed, sp):
