@@ -218,7 +218,11 @@ int gr_plan_create(int nprob, const gr_geometry *const *geoms,
                    const gr_tgv_params *prms, const gr_tgv_state *states,
                    gr_plan **out);
 int gr_plan_destroy(gr_plan *plan);
-/* n_iter iterations of every problem (ku_on_last as in gr_tgv_iterate). */
+/* n_iter iterations of every problem (ku_on_last as in gr_tgv_iterate).
+ * From 2^20 voxel-channels in total an iteration runs as three forks of the
+ * caller's stream (forward projection beside the dual step, backprojection
+ * beside the p half of the primal step, then the u half), each joined back
+ * before the next; smaller plans run the four phases in sequence. */
 int gr_plan_iterate(gr_plan *plan, int n_iter, int ku_on_last, void *stream);
 /* The same through a hipGraph of `block` iterations, captured once on a
  * private stream and replayed on `stream` n_iter / block times; the
